@@ -1,0 +1,281 @@
+"""Python mirror of the reference's host interface over the mdtw C ABI.
+
+The reference (gravitino/cudadtw) is a set of header templates driven like
+this (GPU-dtw.cu:67-96)::
+
+    znormalize(zquery, M); envelope(zquery, W, zlower, zupper, M);
+    cudaMemcpyToSymbol(Czquery/Czlower/Czupper, ...);
+    avg_std<double>(Subject, AvgS, StdS, M, N);
+    prune_cdtw<EUCLIDEAN>(Subject, AvgS, StdS, M, N, W);   // prints result
+
+Here the same steps are ``znormalize``, ``Engine.set_query`` (query and
+envelope into the engine), ``Engine.avg_std`` and ``Engine.prune_cdtw`` /
+``Engine.search``, which *return* the match instead of printing it.  Device
+arrays are torch tensors on the engine's GPU; every call runs on torch's
+current stream of that device, so it orders with surrounding torch work.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ALGO_CDTW, ALGO_ED, ALGO_LDTW, METRIC_L1, METRIC_L2, STATS_FP64,
+                   STATS_REFCOMPAT, check)
+
+__all__ = ["Engine", "SearchResult", "znormalize", "envelope", "band_width",
+           "ALGO_CDTW", "ALGO_ED", "ALGO_LDTW", "METRIC_L1", "METRIC_L2",
+           "STATS_FP64", "STATS_REFCOMPAT"]
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def znormalize(series) -> np.ndarray:
+    """stats.cuh:8-28 znormalize<float>: returns the z-normalised FP32 copy,
+    bit-identical to the reference's host routine (FP32 sums in order)."""
+    x = np.ascontiguousarray(series, dtype=np.float32).copy()
+    check(_lib.load().mdtw_znormalize(_fp(x), x.size), "znormalize")
+    return x
+
+
+def envelope(series, w: int):
+    """bounds.cuh:13-64 envelope: (lower, upper) of a query for band w."""
+    q = np.ascontiguousarray(series, dtype=np.float32)
+    lo = np.empty_like(q)
+    up = np.empty_like(q)
+    check(_lib.load().mdtw_envelope(_fp(q), int(w), _fp(lo), _fp(up), q.size), "envelope")
+    return lo, up
+
+
+def band_width(m: int, percent: float, mode: str = "round") -> int:
+    """Absolute Sakoe-Chiba band for a query of length m.  ``mode="truncate"``
+    is the reference CLI's ``W = M*(P*0.01)`` (GPU-dtw.cu:27)."""
+    if mode == "truncate":
+        return int(m * (int(percent) * 0.01))
+    return int(np.floor(m * percent / 100.0 + 0.5))
+
+
+@dataclass
+class SearchResult:
+    positions: np.ndarray   # int64[k], -1 where fewer than k matches exist
+    distances: np.ndarray   # float32[k]; sqrt of squared cDTW/ED for L2
+    info: dict
+
+    @property
+    def location(self) -> int:
+        return int(self.positions[0])
+
+    @property
+    def distance(self) -> float:
+        return float(self.distances[0])
+
+
+class Engine:
+    """One device's search engine: owns the query state (the reference's
+    __constant__ Czquery/Czlower/Czupper) and reusable scratch."""
+
+    def __init__(self, device: int = 0):
+        import torch
+
+        self._torch = torch
+        self.lib = _lib.load()
+        self.device = int(device)
+        self._dev = torch.device("cuda", self.device)
+        h = C.c_void_p()
+        check(self.lib.mdtw_create(self.device, C.byref(h)), "mdtw_create")
+        self.h = h
+        self.m = 0
+        self.w = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mdtw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- plumbing ----------------------------------------------------------
+    def _stream(self):
+        s = self._torch.cuda.current_stream(self._dev).cuda_stream
+        check(self.lib.mdtw_set_stream(self.h, C.c_void_p(s)), "mdtw_set_stream")
+
+    def _f32(self, t, name):
+        torch = self._torch
+        if not (isinstance(t, torch.Tensor) and t.device == self._dev and t.dtype == torch.float32
+                and t.is_contiguous()):
+            raise TypeError(f"{name}: expected a contiguous float32 tensor on {self._dev}")
+        return C.c_void_p(t.data_ptr())
+
+    def _i32(self, t, name):
+        torch = self._torch
+        if t is None:
+            return None
+        if not (isinstance(t, torch.Tensor) and t.device == self._dev and t.dtype == torch.int32
+                and t.is_contiguous()):
+            raise TypeError(f"{name}: expected a contiguous int32 tensor on {self._dev}")
+        return C.c_void_p(t.data_ptr())
+
+    def empty(self, n, dtype=None):
+        return self._torch.empty(int(n), dtype=dtype or self._torch.float32, device=self._dev)
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        cus = C.c_int32()
+        check(self.lib.mdtw_device_info(self.h, name, 256, C.byref(cus)), "mdtw_device_info")
+        return name.value.decode(), cus.value
+
+    # -- query -------------------------------------------------------------
+    def set_query(self, zquery, w: int, lower=None, upper=None):
+        """cudaMemcpyToSymbol(Czquery/Czlower/Czupper) (GPU-dtw.cu:82-84).
+        zquery must already be z-normalised (``znormalize``)."""
+        q = np.ascontiguousarray(zquery, dtype=np.float32)
+        lo = up = None
+        if lower is not None:
+            lo = np.ascontiguousarray(lower, dtype=np.float32)
+            up = np.ascontiguousarray(upper, dtype=np.float32)
+        self._stream()
+        check(self.lib.mdtw_set_query(self.h, _fp(q), _fp(lo) if lo is not None else None,
+                                      _fp(up) if up is not None else None, q.size, int(w)),
+              "mdtw_set_query")
+        self.m = q.size
+        self.w = int(w)
+
+    # -- statistics and bounds ---------------------------------------------
+    def avg_std(self, series, m: int, mode: int = STATS_FP64, out=None):
+        """stats.cuh:115-142 avg_std: per-window mean and sigma (FP32)."""
+        n = series.numel()
+        L = n - m + 1
+        avg, sd = out if out is not None else (self.empty(max(L, 1)), self.empty(max(L, 1)))
+        self._stream()
+        check(self.lib.mdtw_avg_std(self.h, self._f32(series, "series"), n, int(m),
+                                    self._f32(avg, "avg"), self._f32(sd, "std"), int(mode)),
+              "mdtw_avg_std")
+        return avg, sd
+
+    def _count(self, indices, count, series):
+        if indices is not None:
+            return indices.numel()
+        if count is not None:
+            return int(count)
+        return series.numel() - self.m + 1
+
+    def lb_kim(self, series, avg, sd, indices=None, count=None, metric=METRIC_L2):
+        """bounds.cuh:413-441 lb_kim (all windows, or the gathered indices)."""
+        n = self._count(indices, count, series)
+        out = self.empty(max(n, 1))
+        self._stream()
+        check(self.lib.mdtw_lb_kim(self.h, self._f32(series, "series"), self._f32(avg, "avg"),
+                                   self._f32(sd, "std"), self._f32(out, "out"),
+                                   self._i32(indices, "indices"), n, int(metric)),
+              "mdtw_lb_kim")
+        return out[:n]
+
+    def lb_keogh(self, series, avg, sd, indices=None, count=None, metric=METRIC_L2):
+        """bounds.cuh:443-465 lb_keogh against the query envelope."""
+        n = self._count(indices, count, series)
+        out = self.empty(max(n, 1))
+        self._stream()
+        check(self.lib.mdtw_lb_keogh(self.h, self._f32(series, "series"), self._f32(avg, "avg"),
+                                     self._f32(sd, "std"), self._f32(out, "out"),
+                                     self._i32(indices, "indices"), n, int(metric)),
+              "mdtw_lb_keogh")
+        return out[:n]
+
+    # -- distance profiles -------------------------------------------------
+    def gpu_cdtw(self, series, avg, sd, w: int, indices=None, count=None, metric=METRIC_L2,
+                 wide: bool = False, out=None):
+        """cdtw.cuh:450-487 gpu_cdtw: squared banded cDTW per window."""
+        n = self._count(indices, count, series)
+        if out is None:
+            out = self.empty(max(n, 1))
+        fn = self.lib.mdtw_cdtw_wide if wide else self.lib.mdtw_cdtw
+        self._stream()
+        check(fn(self.h, self._f32(series, "series"), self._f32(avg, "avg"), self._f32(sd, "std"),
+                 self._f32(out, "out"), self._i32(indices, "indices"), n, int(w), int(metric)),
+              "mdtw_cdtw")
+        return out[:n]
+
+    def ldtw(self, series, avg, sd, w: int, count=None, metric=METRIC_L2):
+        """ldtw.cuh:294-391 ztsh2_ldtw: greedy LDTW of every window."""
+        n = self._count(None, count, series)
+        out = self.empty(max(n, 1))
+        self._stream()
+        check(self.lib.mdtw_ldtw(self.h, self._f32(series, "series"), self._f32(avg, "avg"),
+                                 self._f32(sd, "std"), self._f32(out, "out"), n, int(w),
+                                 int(metric)), "mdtw_ldtw")
+        return out[:n]
+
+    def ed(self, series, avg, sd, count=None):
+        """Squared z-normalised Euclidean distance per window (ed.cuh:61-69)."""
+        n = self._count(None, count, series)
+        out = self.empty(max(n, 1))
+        self._stream()
+        check(self.lib.mdtw_ed(self.h, self._f32(series, "series"), self._f32(avg, "avg"),
+                               self._f32(sd, "std"), self._f32(out, "out"), n), "mdtw_ed")
+        return out[:n]
+
+    # -- search ------------------------------------------------------------
+    def search(self, series, w: int, algo: int = ALGO_CDTW, metric: int = METRIC_L2, k: int = 1,
+               excl: int = 0, avg=None, sd=None, stats_mode: int = STATS_FP64) -> SearchResult:
+        """Best (k = 1) or top-k matches (cdtw.cuh:510-592, ed.cuh:71-153)."""
+        pos = np.full(k, -1, dtype=np.int64)
+        dist = np.full(k, np.inf, dtype=np.float32)
+        info = _lib.SearchInfo()
+        self._stream()
+        check(self.lib.mdtw_search(
+            self.h, self._f32(series, "series"), series.numel(),
+            self._f32(avg, "avg") if avg is not None else None,
+            self._f32(sd, "std") if sd is not None else None, int(w), int(algo), int(metric),
+            int(stats_mode), int(k), int(excl), pos.ctypes.data_as(C.POINTER(C.c_int64)),
+            _fp(dist), C.byref(info)), "mdtw_search")
+        return SearchResult(pos, dist, info.as_dict())
+
+    def prune_cdtw(self, subject, avg, sd, w: int, metric: int = METRIC_L2):
+        """cdtw.cuh:510-592 prune_cdtw: returns (location, distance) instead
+        of printing "Location:" / "Distance:" (cdtw.cuh:506-507)."""
+        r = self.search(subject, w, ALGO_CDTW, metric, 1, 0, avg, sd)
+        return r.location, r.distance
+
+    def calculate_ed(self, subject, avg=None, sd=None):
+        """ed.cuh:71-153 calculate_ed: (location, distance) of the best
+        z-normalised Euclidean match."""
+        r = self.search(subject, 0, ALGO_ED, METRIC_L2, 1, 0, avg, sd)
+        return r.location, r.distance
+
+    # -- primitives --------------------------------------------------------
+    def threshold(self, values, indices, thr: float) -> int:
+        """cub_util.cuh:154-172 threshold: stable in-place compaction of the
+        indices whose value is <= thr; returns the new length."""
+        cnt = C.c_int64()
+        self._stream()
+        check(self.lib.mdtw_threshold(self.h, self._f32(values, "values"),
+                                      self._i32(indices, "indices"), values.numel(),
+                                      C.c_float(thr), C.byref(cnt)), "mdtw_threshold")
+        return cnt.value
+
+    def argmin(self, values):
+        """Minimum value and its lowest index (cub_util.cuh:7-27 reduce +
+        cdtw.cuh:493-508 report_result)."""
+        idx = C.c_int64()
+        val = C.c_float()
+        self._stream()
+        check(self.lib.mdtw_argmin(self.h, self._f32(values, "values"), values.numel(),
+                                   C.byref(idx), C.byref(val)), "mdtw_argmin")
+        return idx.value, val.value
+
+    def synchronize(self):
+        check(self.lib.mdtw_synchronize(self.h), "mdtw_synchronize")

@@ -1,0 +1,249 @@
+// cdtw.hip -- banded cDTW dispatch and the wide-band kernel.
+//
+// W <= kBandMaxW (L2 metric): the register-band kernel (cdtw_band.hpp), one
+// lane per window.  Wider bands (up to kMaxBandW) and the L1 metric beyond
+// the band kernel's range use the anti-diagonal kernel below.
+//
+// Anti-diagonal kernel.  One wave per window.  Diagonal k = i + j holds the
+// cells with offset d = i - j in [-W, W], d = k (mod 2); slot p of diagonal k
+// has d = -W + 2p + par(k), par(k) = (k + W) & 1, so p = 0..W.  Lane l owns
+// slots l*C .. l*C + C - 1 (slots past W are padding, forced to +inf).  With
+// this layout the predecessors of slot p are: diagonal k-2 slot p (i-1, j-1);
+// diagonal k-1 slots p-1 and p (par 0) or p and p+1 (par 1), so a step needs
+// one cross-lane value.  Query values move one slot down and stream values
+// one slot up every other step; fresh query values enter at the top physical
+// slot, fresh z-normalised stream values at slot 0.  Out-of-matrix query
+// values are +inf and stream values -inf, so q - z is +inf there and never
+// NaN.  The reference runs the same diagonal recurrence with a barrier per
+// diagonal and shared-memory exchange (cdtw.cuh:261-363); here a diagonal is
+// one wave, exchanged through cross-lane shuffles.
+#include "cdtw_band.hpp"
+
+namespace mdtw {
+namespace {
+
+struct WideScalars {
+    int32_t m;
+    int32_t w;
+    int32_t idx_count_from_ctrl;
+    int64_t count;
+    int64_t base;
+    int64_t stride;
+    int32_t mode;
+    int32_t persistent;
+};
+
+constexpr int kWideCheck = 32;  // steps between abandon checks
+
+__device__ __forceinline__ float wave_min_f32(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+template <int C, bool L2>
+__device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
+                         const float* __restrict__ q, int32_t m, int32_t W, int mode,
+                         const Ctrl* ctrl, bool& alive) {
+    const int lane = threadIdx.x;
+    const int P = W + 1;
+    float pad0[C], pad1[C], A1[C], A2[C], Qv[C], Zv[C];
+    const int pfin = W >> 1;
+    const int par0 = W & 1;
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int p = lane * C + c;
+        pad0[c] = p < P ? 0.0f : kInf;
+        pad1[c] = p < P - 1 ? 0.0f : kInf;
+        A1[c] = kInf;
+        A2[c] = p == pfin ? 0.0f : kInf;  // D[-1][-1] on diagonal -2
+        const int d = -W + 2 * p + par0;  // diagonal 0: i = d/2, j = -d/2
+        const int i = d / 2, j = -i;
+        Qv[c] = (i >= 0 && i < m) ? q[i] : kInf;
+        Zv[c] = (j >= 0 && j < m) ? (x[j] - mu) / sg : -kInf;
+    }
+    int jb = 0;
+    float zb = lane < m ? (x[lane] - mu) / sg : -kInf;  // z[jb + lane]
+    const bool abandon = mode == kModeBest || mode == kModeCollect;
+    const int K = 2 * m - 1;
+    for (int k = 0; k < K; ++k) {
+        const int par = (k + W) & 1;
+        float nw[C];
+        if (par == 0) {
+            float in = __shfl_up(A1[C - 1], 1, kWave);
+            if (lane == 0) in = kInf;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float up = c > 0 ? A1[c - 1] : in;
+                const float mn = fminf(fminf(A2[c], up), A1[c]);
+                const float dd = Qv[c] - Zv[c];
+                const float v = L2 ? fmaf(dd, dd, mn) : fabsf(dd) + mn;
+                nw[c] = fmaxf(v, pad0[c]);
+            }
+        } else {
+            float in = __shfl_down(A1[0], 1, kWave);
+            if (lane == kWave - 1) in = kInf;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float lf = c + 1 < C ? A1[c + 1] : in;
+                const float mn = fminf(fminf(A2[c], A1[c]), lf);
+                const float dd = Qv[c] - Zv[c];
+                const float v = L2 ? fmaf(dd, dd, mn) : fabsf(dd) + mn;
+                nw[c] = fmaxf(v, pad1[c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            A2[c] = A1[c];
+            A1[c] = nw[c];
+        }
+        if (par == 0) {
+            // i advances: slot p takes slot p+1's query value.
+            const float in = __shfl_down(Qv[0], 1, kWave);
+#pragma unroll
+            for (int c = 0; c + 1 < C; ++c) Qv[c] = Qv[c + 1];
+            const int iq = (k - W) / 2 + kWave * C;
+            const float fresh = (iq >= 0 && iq < m) ? q[iq] : kInf;
+            Qv[C - 1] = lane == kWave - 1 ? fresh : in;
+        } else {
+            // j advances: slot p takes slot p-1's stream value.
+            const float in = __shfl_up(Zv[C - 1], 1, kWave);
+#pragma unroll
+            for (int c = C - 1; c > 0; --c) Zv[c] = Zv[c - 1];
+            const int iz = (k + W + 1) / 2;
+            float fresh = -kInf;
+            if (iz < m) {
+                if (iz >= jb + kWave) {
+                    jb = iz & ~(kWave - 1);
+                    zb = jb + lane < m ? (x[jb + lane] - mu) / sg : -kInf;
+                }
+                fresh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zb), iz - jb));
+            }
+            Zv[0] = lane == 0 ? fresh : in;
+        }
+        if (abandon && (k % kWideCheck) == kWideCheck - 1) {
+            float cm = kInf;
+#pragma unroll
+            for (int c = 0; c < C; ++c) cm = fminf(cm, fminf(A1[c], A2[c]));
+            cm = wave_min_f32(cm);
+            const float thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
+            if (cm > thr) {
+                alive = false;
+                return kInf;
+            }
+        }
+    }
+    // D[M-1][M-1]: diagonal 2M-2, d = 0 -> slot W >> 1.
+    float r = A1[0];
+#pragma unroll
+    for (int c = 1; c < C; ++c)
+        if (c == pfin % C) r = A1[c];
+    return __shfl(r, pfin / C, kWave);
+}
+
+template <int C, bool L2>
+__global__ __launch_bounds__(64) void wide_kernel(
+    const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
+    const float* __restrict__ q, const int32_t* __restrict__ idx, float* __restrict__ out,
+    unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, WideScalars a) {
+    const int lane = threadIdx.x;
+    int64_t count = a.count;
+    if (a.idx_count_from_ctrl)
+        count = static_cast<int64_t>(
+            __hip_atomic_load(&ctrl->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    unsigned long long n_dp = 0, n_ab = 0;
+    auto next = [&]() -> int64_t {
+        unsigned long long b = 0;
+        if (lane == 0) b = atomicAdd(&ctrl->work, 1ull);
+        return static_cast<int64_t>(__shfl(b, 0, kWave));
+    };
+    int64_t g = a.persistent ? next() : static_cast<int64_t>(blockIdx.x);
+    while (g < count) {
+        const int64_t w = idx ? static_cast<int64_t>(idx[g]) : a.base + g * a.stride;
+        bool alive = true;
+        const float d = wide_dp<C, L2>(s + w, avg[w], sd[w], q, a.m, a.w, a.mode, ctrl, alive);
+        if (lane == 0) {
+            if (a.mode == kModeProfile || a.mode == kModeSeed) out[g] = d;
+            if ((a.mode == kModeBest || a.mode == kModeSeed) && alive)
+                atomicMin(&ctrl->best, pack_key(d, static_cast<uint32_t>(w)));
+            if (a.mode == kModeCollect && alive && d <= load_thr(ctrl)) {
+                const unsigned long long r = atomicAdd(&ctrl->nres, 1ull);
+                res[r] = pack_key(d, static_cast<uint32_t>(w));
+            }
+        }
+        n_dp += 1;
+        n_ab += alive ? 0 : 1;
+        if (!a.persistent) break;
+        g = next();
+    }
+    if (lane == 0 && a.mode != kModeProfile) {
+        if (n_dp) atomicAdd(&ctrl->dp, n_dp);
+        if (n_ab) atomicAdd(&ctrl->abandoned, n_ab);
+    }
+}
+
+template <int C, bool L2>
+int wide_launch(hipStream_t st, unsigned grid, const BandArgs& a, int32_t w) {
+    WideScalars sc{a.m, w, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode,
+                   a.persistent};
+    hipLaunchKernelGGL((wide_kernel<C, L2>), dim3(grid), dim3(kWave), 0, st, a.s, a.avg, a.sd,
+                       a.qx, a.idx, a.out, a.res, a.ctrl, sc);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+int wide_dispatch(hipStream_t st, unsigned grid, const BandArgs& a, int32_t w, bool l2) {
+    const int slots = w + 1;
+    if (slots <= 64)
+        return l2 ? wide_launch<1, true>(st, grid, a, w) : wide_launch<1, false>(st, grid, a, w);
+    if (slots <= 128)
+        return l2 ? wide_launch<2, true>(st, grid, a, w) : wide_launch<2, false>(st, grid, a, w);
+    if (slots <= 256)
+        return l2 ? wide_launch<4, true>(st, grid, a, w) : wide_launch<4, false>(st, grid, a, w);
+    if (slots <= 512)
+        return l2 ? wide_launch<8, true>(st, grid, a, w) : wide_launch<8, false>(st, grid, a, w);
+    return l2 ? wide_launch<16, true>(st, grid, a, w) : wide_launch<16, false>(st, grid, a, w);
+}
+
+BandLaunchFn band_lookup(int w, bool l2) {
+    if (w < 0 || w > kBandMaxW) return nullptr;
+    if (w < 30) return band_table_0(w, l2);
+    if (w < 60) return band_table_1(w, l2);
+    if (w < 90) return band_table_2(w, l2);
+    return band_table_3(w, l2);
+}
+
+}  // namespace
+
+// Band kernels need the padded query (qx); the wide kernel the plain query.
+// The caller passes both through BandArgs::qx / q_plain.
+int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain, int32_t w,
+              int32_t metric, bool force_wide) {
+    const bool l2 = metric != 0;
+    const int64_t count_hint = a.idx_count_from_ctrl ? -1 : a.count;
+    if (!a.idx_count_from_ctrl && a.count <= 0) return MDTW_OK;
+    BandLaunchFn fn = force_wide ? nullptr : band_lookup(w, l2);
+    if (fn) {
+        unsigned grid;
+        if (a.persistent)
+            grid = static_cast<unsigned>(cus * 16);
+        else
+            grid = static_cast<unsigned>((count_hint + kWave - 1) / kWave);
+        return fn(st, grid, a);
+    }
+    if (w > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
+    BandArgs b = a;
+    b.qx = q_plain;
+    unsigned grid;
+    if (a.persistent)
+        grid = static_cast<unsigned>(cus * 32);
+    else
+        grid = static_cast<unsigned>(count_hint);
+    return wide_dispatch(st, grid, b, w, l2);
+}
+
+}  // namespace mdtw

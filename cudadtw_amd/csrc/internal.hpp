@@ -1,0 +1,224 @@
+// internal.hpp -- shared declarations of the mdtw HIP library (gfx950).
+//
+// Floating-point policy (all translation units): -ffp-contract=off, packed
+// FP32 disabled; every fused multiply-add is an explicit fmaf, division and
+// sqrt are IEEE (correctly rounded).  That fixes the rounding sequence of each
+// kernel so it can be compared bit-for-bit with the CPU restatement.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "mdtw.h"
+
+namespace mdtw {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr int kWave = 64;
+
+// Registers-per-lane band kernel covers W <= kRegBandMaxW; wider bands use the
+// anti-diagonal kernel, which covers W <= kMaxBandW (the reference's own limit
+// is W + 2 <= 1024 threads, cdtw.cuh:459).
+constexpr int kRegBandMaxW = 57;
+constexpr int kMaxBandW = 1023;
+
+// Control block in device memory, one per engine.
+struct Ctrl {
+    unsigned long long best;     // (float bits << 32) | window index
+    unsigned long long count;    // candidates appended
+    unsigned long long work;     // persistent-kernel work cursor
+    unsigned long long nres;     // results appended (top-k)
+    unsigned long long kim;      // pruned by LB_Kim
+    unsigned long long keogh;    // pruned by LB_Keogh
+    unsigned long long dp;       // DP runs started
+    unsigned long long abandoned;// DP runs stopped early
+    float thr;                   // fixed threshold (top-k), squared distance
+    float pad[7];
+};
+
+}  // namespace mdtw
+
+struct mdtw_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    char name[256] = {0};
+
+    // Query state: the reference's Czquery/Czlower/Czupper (cdtw.cuh:9-11).
+    int32_t m = 0;
+    int32_t w_env = -1;
+    bool has_query = false;
+    float* d_q = nullptr;
+    float* d_lo = nullptr;
+    float* d_up = nullptr;
+    std::vector<float> h_q, h_lo, h_up;
+    // Query padded with +inf for the register-band kernel, built per band.
+    float* d_qx = nullptr;
+    size_t qx_cap = 0;
+    int32_t w_qx = -1;
+
+    // Growing scratch, reused across calls (no per-call malloc).
+    char* d_ws = nullptr;
+    size_t ws_cap = 0;
+
+    mdtw::Ctrl* d_ctrl = nullptr;
+    mdtw::Ctrl* h_ctrl = nullptr;  // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace mdtw {
+
+void set_last_hip_error(hipError_t e);
+int ws_reserve(mdtw_engine* e, size_t bytes, void** out);
+int upload(mdtw_engine* e, void* dst, const void* src, size_t bytes);
+void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m);
+int ensure_envelope(mdtw_engine* e, int32_t w);
+int ensure_qx(mdtw_engine* e, int32_t w);
+
+// Sub-allocator over one reserved workspace span (256-byte aligned pieces).
+struct Carve {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t n) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += n * sizeof(T);
+        return p;
+    }
+};
+inline size_t carve_size(std::initializer_list<size_t> bytes) {
+    size_t t = 0;
+    for (size_t b : bytes) t = ((t + 255) & ~size_t(255)) + b;
+    return t + 256;
+}
+
+// ---- kernel interface shared by the DP-style kernels ----------------------
+enum BandMode : int {
+    kModeProfile = 0,  // out[g] = distance of window g
+    kModeBest = 1,     // atomicMin (distance, index) into ctrl->best; abandon above it
+    kModeCollect = 2,  // append keys with distance <= ctrl->thr to res; abandon above it
+    kModeSeed = 3,     // out[g] = distance, and atomicMin into ctrl->best
+};
+
+// Windows are idx[g] (idx non-null; count read from ctrl->count on the device
+// when idx_count_from_ctrl) or base + g * stride, for g < count.
+struct BandArgs {
+    const float* s = nullptr;
+    const float* avg = nullptr;
+    const float* sd = nullptr;
+    const float* qx = nullptr;  // band kernel: padded query; others: plain query
+    int32_t m = 0;
+    const int32_t* idx = nullptr;
+    int32_t idx_count_from_ctrl = 0;
+    int64_t count = 0;
+    int64_t base = 0;
+    int64_t stride = 1;
+    int32_t mode = kModeProfile;
+    int32_t persistent = 0;
+    Ctrl* ctrl = nullptr;
+    unsigned long long* res = nullptr;
+    float* out = nullptr;
+};
+
+struct BandScalars {
+    int32_t m;
+    int32_t idx_count_from_ctrl;
+    int64_t count;
+    int64_t base;
+    int64_t stride;
+    int32_t mode;
+    int32_t persistent;
+};
+
+// ---- kernel launchers (each in its own translation unit) -----------------
+// stats.hip
+int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,
+                      float* avg, float* sd);
+int launch_stats_refcompat(mdtw_engine* e, const float* s, int64_t n, int32_t m,
+                           float* avg, float* sd);
+// bounds.hip
+int launch_lb_kim(hipStream_t st, const float* s, const float* avg,
+                  const float* sd, const float* q, int32_t m,
+                  const int32_t* idx, int64_t count, int32_t metric, float* out);
+int launch_lb_keogh(hipStream_t st, const float* s, const float* avg,
+                    const float* sd, const float* lo, const float* up, int32_t m,
+                    const int32_t* idx, int64_t count, int32_t metric, float* out);
+// Search cascade: LB_Kim then LB_Keogh on every window 0..L-1; survivors
+// (bound not above the limit of ctrl->best, or of ctrl->thr) go to cand.
+int launch_cascade(hipStream_t st, const float* s, const float* avg,
+                   const float* sd, const float* q, const float* lo,
+                   const float* up, int32_t m, int64_t L, int32_t metric,
+                   bool use_best, Ctrl* ctrl, int32_t* cand);
+// cdtw.hip: band kernel (qx = padded query) or, for wide bands, the
+// anti-diagonal kernel (q_plain).
+int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain,
+              int32_t w, int32_t metric, bool force_wide);
+// ldtw.hip (a.qx = plain query)
+int launch_ldtw(hipStream_t st, int cus, const BandArgs& a, int32_t w, int32_t metric);
+// ed.hip (a.qx = plain query; no persistent mode)
+int launch_ed(hipStream_t st, int cus, const BandArgs& a);
+// select.hip
+// best_init: kBestNone for searches (key of +inf, so load_best_dist is +inf
+// until a distance lands), ~0ull for the signed-order argmin.
+constexpr unsigned long long kBestNone = (0x7f800000ull << 32) | 0xffffffffull;
+int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init);
+int launch_thr_from_best(hipStream_t st, Ctrl* ctrl);
+int launch_argmin(hipStream_t st, const float* v, int64_t count, Ctrl* ctrl);
+size_t threshold_ws_ints(int64_t count);
+int launch_threshold(hipStream_t st, const float* v, int32_t* idx, int64_t count,
+                     float thr, int32_t* tmp, int32_t** d_total);
+// ctrl->thr = k-th smallest of nseeds values (+inf if fewer than k).
+int launch_seed_threshold(hipStream_t st, const float* seeds, int64_t nseeds,
+                          int32_t k, Ctrl* ctrl);
+constexpr int64_t kSeedMaxCount = 16384;
+
+}  // namespace mdtw
+
+// ---- device helpers -------------------------------------------------------
+namespace mdtw {
+
+__device__ __forceinline__ unsigned long long pack_key(float d, uint32_t idx) {
+    // Non-negative floats (and +inf) order like their bit patterns.
+    return (static_cast<unsigned long long>(__float_as_uint(d)) << 32) | idx;
+}
+__device__ __forceinline__ float key_dist(unsigned long long k) {
+    return __uint_as_float(static_cast<uint32_t>(k >> 32));
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float load_best_dist(const Ctrl* c) {
+    unsigned long long k = __hip_atomic_load(&c->best, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    return key_dist(k);
+}
+__device__ __forceinline__ float load_thr(const Ctrl* c) {
+    return __hip_atomic_load(&c->thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pruning limit that keeps pruning exact.  A computed lower bound can exceed
+// the computed distance of the same window by rounding: the bound sums m
+// rounded terms, the DP rounds along a path of up to 2m cells, and the bound
+// kernels z-normalise with a reciprocal (<= 1.5 ulp per value; summed over a
+// window that perturbs an L2 bound by at most 1.8e-7 * sqrt(m * bound) by
+// Cauchy-Schwarz with sum z^2 = m, an L1 bound by at most 1e-7 * m).  A
+// window is pruned only when its bound exceeds prune_limit(threshold); the
+// limit is monotone, so a partial sum above it also prunes (early abandon).
+__device__ __forceinline__ float prune_limit(float thr, int32_t m, bool l2) {
+    const float rel = thr * (float)(3 * m + 8) * 6.0e-8f;
+    const float absl = l2 ? 2.5e-7f * sqrtf((float)m * thr) : 1.2e-7f * (float)m;
+    return thr + rel + absl + 1e-30f;
+}
+
+}  // namespace mdtw

@@ -1,0 +1,166 @@
+// ldtw.hip -- greedy "locally best" band-constrained DTW (LDTW), one lane
+// per window.
+//
+// Restates ztsh2_ldtw / z_serial_ldtw (ldtw.cuh:74-127, :294-391): from cell
+// (0, 0) the path repeatedly steps to the cheapest of diagonal, down and right
+// (band |i - j| <= W checked for down/right, INFTY = 1e10 for moves that are
+// not allowed), with the reference's tie rules: diagonal only if strictly
+// below both others, then down if not above diagonal and strictly below
+// right, else right.  The cost is the sum of the visited cells' local costs.
+// z-normalisation is (x - mu) / sigma (IEEE division), squares and sums are
+// separately rounded (no contraction), so the value is bit-exact with the CPU
+// restatement.  Each z-normalised stream value is computed once and carried
+// while the path stays on its column.
+//
+// Search modes abandon a wave once every lane's partial cost (which only
+// grows) exceeds the threshold.
+#include "cdtw_band.hpp"
+
+namespace mdtw {
+namespace {
+
+constexpr float kLdtwInf = 1e10f;  // ldtw.cuh:8 INFTY
+constexpr int kLdtwCheck = 32;
+
+template <bool L2>
+__device__ __forceinline__ float lcost(float t) {
+    if constexpr (L2)
+        return t * t;
+    else
+        return t < 0 ? -t : t;
+}
+
+template <bool L2>
+__device__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
+                             const float* __restrict__ q, int32_t M, int32_t W, int mode,
+                             const Ctrl* ctrl, bool& alive) {
+    int32_t i = 0, j = 0;
+    float qi = q[0], qi1 = M > 1 ? q[1] : 0.0f;
+    float zj = (x[0] - mu) / sg;
+    float zj1 = M > 1 ? (x[1] - mu) / sg : 0.0f;
+    float d = lcost<L2>(qi - zj);
+    const bool abandon = mode == kModeBest || mode == kModeCollect;
+    int step = 0;
+    // A valid greedy path reaches (M-1, M-1) in at most 2M-2 moves (every
+    // move advances i or j and the path never leaves the band); the cap and
+    // the clamps below only bound the loop and the indices.
+    const int max_steps = 2 * M - 2;
+    while ((i + 1 != M || j + 1 != M) && step < max_steps) {
+        float diag = kLdtwInf, rght = kLdtwInf, down = kLdtwInf;
+        const bool vi = i + 1 < M, vj = j + 1 < M;
+        if (vi && vj) diag = lcost<L2>(qi1 - zj1);
+        if (vi && abs(i - j + 1) <= W) down = lcost<L2>(qi1 - zj);
+        if (vj && abs(j - i + 1) <= W) rght = lcost<L2>(qi - zj1);
+        const bool dsd = diag < down;
+        const bool dsr = diag < rght;
+        const bool dnr = down < rght;
+        bool adv_i, adv_j;
+        if (dsd && dsr) {
+            d += diag;
+            adv_i = true;
+            adv_j = true;
+        } else if (!dsd && dnr) {
+            d += down;
+            adv_i = true;
+            adv_j = false;
+        } else {
+            d += rght;
+            adv_i = false;
+            adv_j = true;
+        }
+        if (adv_i && i + 1 < M) {
+            ++i;
+            qi = qi1;
+            qi1 = i + 1 < M ? q[i + 1] : 0.0f;
+        }
+        if (adv_j && j + 1 < M) {
+            ++j;
+            zj = zj1;
+            zj1 = j + 1 < M ? (x[j + 1] - mu) / sg : 0.0f;
+        }
+        ++step;
+        if (abandon && (step % kLdtwCheck) == 0) {
+            const float thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
+            alive = alive && !(d > thr);
+            if (!__any(alive)) return kInf;
+        }
+    }
+    return d;
+}
+
+__device__ __forceinline__ uint32_t rank_in(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+template <bool L2>
+__global__ __launch_bounds__(64) void ldtw_kernel(
+    const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
+    const float* __restrict__ q, const int32_t* __restrict__ idx, float* __restrict__ out,
+    unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, BandScalars a, int32_t W) {
+    const int lane = threadIdx.x;
+    int64_t count = a.count;
+    if (a.idx_count_from_ctrl)
+        count = static_cast<int64_t>(
+            __hip_atomic_load(&ctrl->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    unsigned long long n_dp = 0, n_ab = 0;
+    int64_t base = a.persistent ? band_next(ctrl, lane)
+                                : static_cast<int64_t>(blockIdx.x) * kWave;
+    while (base < count) {
+        const int64_t g = base + lane;
+        const bool valid = g < count;
+        const int64_t gc = valid ? g : count - 1;
+        const int64_t w = idx ? static_cast<int64_t>(idx[gc]) : a.base + gc * a.stride;
+        bool alive = valid;
+        const float d = ldtw_window<L2>(s + w, avg[w], sd[w], q, a.m, W, a.mode, ctrl, alive);
+        if (a.mode == kModeProfile || a.mode == kModeSeed) {
+            if (valid) out[g] = d;
+        }
+        if (a.mode == kModeBest || a.mode == kModeSeed) {
+            unsigned long long key = alive ? pack_key(d, static_cast<uint32_t>(w)) : ~0ull;
+            key = wave_min_u64(key);
+            if (lane == 0 && key != ~0ull) atomicMin(&ctrl->best, key);
+        } else if (a.mode == kModeCollect) {
+            const bool keep = alive && d <= load_thr(ctrl);
+            const unsigned long long km = __ballot(keep);
+            if (km) {
+                unsigned long long rb = 0;
+                if (lane == 0)
+                    rb = atomicAdd(&ctrl->nres, static_cast<unsigned long long>(__popcll(km)));
+                rb = __shfl(rb, 0, kWave);
+                if (keep) res[rb + rank_in(km)] = pack_key(d, static_cast<uint32_t>(w));
+            }
+        }
+        n_dp += __popcll(__ballot(valid));
+        n_ab += __popcll(__ballot(valid && !alive));
+        if (!a.persistent) break;
+        base = band_next(ctrl, lane);
+    }
+    if (lane == 0 && a.mode != kModeProfile) {
+        if (n_dp) atomicAdd(&ctrl->dp, n_dp);
+        if (n_ab) atomicAdd(&ctrl->abandoned, n_ab);
+    }
+}
+
+}  // namespace
+
+int launch_ldtw(hipStream_t st, int cus, const BandArgs& a, int32_t w, int32_t metric) {
+    if (!a.idx_count_from_ctrl && a.count <= 0) return MDTW_OK;
+    BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent};
+    const unsigned grid = a.persistent ? static_cast<unsigned>(cus * 16)
+                                       : static_cast<unsigned>((a.count + kWave - 1) / kWave);
+    if (metric)
+        hipLaunchKernelGGL(ldtw_kernel<true>, dim3(grid), dim3(kWave), 0, st, a.s, a.avg, a.sd,
+                           a.qx, a.idx, a.out, a.res, a.ctrl, sc, w);
+    else
+        hipLaunchKernelGGL(ldtw_kernel<false>, dim3(grid), dim3(kWave), 0, st, a.s, a.avg, a.sd,
+                           a.qx, a.idx, a.out, a.res, a.ctrl, sc, w);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+}  // namespace mdtw

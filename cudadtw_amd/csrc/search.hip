@@ -1,0 +1,258 @@
+// search.hip -- mdtw_search: best or top-k matches of the engine's query.
+//
+// Replaces prune_cdtw (cdtw.cuh:510-592), calculate_ed (ed.cuh:71-153) and
+// the GPU-ldtw.cu driver.  The reference sorts all windows by LB_Kim, then
+// runs a host loop over doubling chunks (LB_Keogh, compaction, DP, reduce)
+// with several host-device synchronisations and mallocs per chunk.  Here the
+// whole search is a fixed sequence of launches on one stream with no host
+// round trip until the answer:
+//   1. window statistics (unless supplied),
+//   2. seed: exact distances of up to 16384 evenly strided windows, which
+//      sets the bound (k = 1: best seed; k > 1: k-th smallest seed, valid
+//      because the seeds are >= 2*excl apart, so no pick can exclude two),
+//   3. cascade: LB_Kim then LB_Keogh over every window, survivors appended
+//      to a candidate list,
+//   4. persistent DP over the candidates with early abandoning,
+//      k = 1: atomicMin of (distance, index) keys (lowest index wins ties);
+//      k > 1: every candidate at or below the bound is kept as a key,
+//   5. k > 1: the kept keys are sorted and picked greedily with the
+//      exclusion zone on the host (they are few).
+// Pruning never changes the answer: windows are dropped only when a bound
+// strictly exceeds prune_limit(threshold) (rounding-aware), and a DP run is
+// abandoned only when its running column minimum strictly exceeds the
+// threshold.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cdtw_band.hpp"
+
+using namespace mdtw;
+
+namespace {
+
+constexpr int64_t kSeedTarget = 8192;
+
+struct SeedPlan {
+    int64_t count;
+    int64_t stride;
+};
+
+SeedPlan plan_seeds(int64_t L, int32_t k, int64_t excl) {
+    int64_t stride = std::max<int64_t>(1, L / kSeedTarget);
+    if (k > 1) stride = std::max<int64_t>(stride, 2 * std::max<int64_t>(excl, 0));
+    int64_t count = (L - 1) / stride + 1;
+    if (count > kSeedMaxCount) {
+        stride = (L + kSeedMaxCount - 1) / kSeedMaxCount;
+        if (k > 1) stride = std::max<int64_t>(stride, 2 * std::max<int64_t>(excl, 0));
+        count = (L - 1) / stride + 1;
+    }
+    return {count, stride};
+}
+
+int hip_check(hipError_t err) {
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+// One DP-style launch for the selected algorithm.
+int run_algo(mdtw_engine* e, int32_t algo, const BandArgs& a, int32_t w, int32_t metric) {
+    if (algo == MDTW_ALGO_CDTW) return launch_dp(e->stream, e->cus, a, e->d_q, w, metric, false);
+    BandArgs b = a;
+    b.qx = e->d_q;
+    if (algo == MDTW_ALGO_LDTW) return launch_ldtw(e->stream, e->cus, b, w, metric);
+    return launch_ed(e->stream, e->cus, b);
+}
+
+}  // namespace
+
+extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, const float* d_avg,
+                           const float* d_std, int32_t w, int32_t algo, int32_t metric,
+                           int32_t stats_mode, int32_t k, int64_t excl, int64_t* h_pos,
+                           float* h_dist, mdtw_search_info* info) {
+    if (!e || !d_series || !h_pos || !h_dist || w < 0) return w < 0 ? MDTW_ERR_BAND
+                                                                    : MDTW_ERR_INVALID_ARGUMENT;
+    if (!e->has_query) return MDTW_ERR_NO_QUERY;
+    if ((d_avg == nullptr) != (d_std == nullptr)) return MDTW_ERR_INVALID_ARGUMENT;
+    if (algo != MDTW_ALGO_CDTW && algo != MDTW_ALGO_ED && algo != MDTW_ALGO_LDTW)
+        return MDTW_ERR_INVALID_ARGUMENT;
+    if (k < 1 || k > MDTW_MAX_K) return MDTW_ERR_K;
+    if (metric != MDTW_METRIC_L1 && metric != MDTW_METRIC_L2) return MDTW_ERR_INVALID_ARGUMENT;
+    if (algo == MDTW_ALGO_ED && metric != MDTW_METRIC_L2) return MDTW_ERR_UNSUPPORTED;
+    const int32_t m = e->m;
+    if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
+    const int64_t L = n - m + 1;
+    if (L > 0xffffffffll) return MDTW_ERR_UNSUPPORTED;
+    const int32_t wc = std::min(w, m - 1);
+    if (algo == MDTW_ALGO_CDTW && wc > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
+    for (int32_t i = 0; i < k; ++i) {
+        h_pos[i] = -1;
+        h_dist[i] = INFINITY;
+    }
+    int dev_prev = -1;
+    if (hipGetDevice(&dev_prev) == hipSuccess && dev_prev != e->device) (void)hipSetDevice(e->device);
+    else dev_prev = -1;
+    struct Restore {
+        int d;
+        ~Restore() {
+            if (d >= 0) (void)hipSetDevice(d);
+        }
+    } restore{dev_prev};
+
+    // Query-side state first (host work and small uploads, outside timing).
+    int rc = MDTW_OK;
+    if (algo != MDTW_ALGO_ED) {
+        rc = ensure_envelope(e, wc);
+        if (rc == MDTW_OK && algo == MDTW_ALGO_CDTW) rc = ensure_qx(e, wc);
+        if (rc != MDTW_OK) return rc;
+    }
+
+    const SeedPlan sp = plan_seeds(L, k, excl);
+    const bool own_stats = d_avg == nullptr;
+    const bool topk = k > 1;
+    const size_t bytes = carve_size({own_stats ? sizeof(float) * L : 0,
+                                     own_stats ? sizeof(float) * L : 0,
+                                     sizeof(int32_t) * L,
+                                     topk ? sizeof(unsigned long long) * L : 0,
+                                     sizeof(float) * sp.count});
+    void* ws = nullptr;
+    rc = ws_reserve(e, bytes, &ws);
+    if (rc != MDTW_OK) return rc;
+    Carve cv{static_cast<char*>(ws)};
+    float* avg = own_stats ? cv.take<float>(L) : nullptr;
+    float* sd = own_stats ? cv.take<float>(L) : nullptr;
+    int32_t* cand = cv.take<int32_t>(L);
+    unsigned long long* res = topk ? cv.take<unsigned long long>(L) : nullptr;
+    float* seeds = cv.take<float>(sp.count);
+    if (!own_stats) {
+        avg = const_cast<float*>(d_avg);
+        sd = const_cast<float*>(d_std);
+    }
+
+    hipStream_t st = e->stream;
+    rc = hip_check(hipEventRecord(e->ev0, st));
+    if (rc == MDTW_OK) rc = launch_ctrl_reset(st, e->d_ctrl, kBestNone);
+    if (rc == MDTW_OK && own_stats) {
+        rc = stats_mode == MDTW_STATS_REFCOMPAT
+                 ? launch_stats_refcompat(e, d_series, n, m, avg, sd)
+                 : launch_stats_fp64(st, d_series, n, m, avg, sd);
+    }
+    if (rc != MDTW_OK) return rc;
+
+    BandArgs a;
+    a.s = d_series;
+    a.avg = avg;
+    a.sd = sd;
+    a.qx = e->d_qx;
+    a.m = m;
+    a.ctrl = e->d_ctrl;
+    a.res = res;
+
+    // 2. seed
+    BandArgs sa = a;
+    sa.count = sp.count;
+    sa.base = 0;
+    sa.stride = sp.stride;
+    sa.mode = topk ? kModeSeed : kModeBest;
+    sa.out = seeds;
+    rc = run_algo(e, algo, sa, wc, metric);
+    if (rc == MDTW_OK && topk) rc = launch_seed_threshold(st, seeds, sp.count, k, e->d_ctrl);
+    if (rc != MDTW_OK) return rc;
+
+    if (algo == MDTW_ALGO_ED) {
+        // ED is cheap per window and its own early abandon is the filter.
+        BandArgs fa = a;
+        fa.count = L;
+        fa.mode = topk ? kModeCollect : kModeBest;
+        rc = run_algo(e, algo, fa, wc, metric);
+    } else if (m >= 11) {
+        // 3. cascade, 4. DP over survivors.
+        rc = launch_cascade(st, d_series, avg, sd, e->d_q, e->d_lo, e->d_up, m, L, metric,
+                            !topk, e->d_ctrl, cand);
+        if (rc == MDTW_OK) {
+            BandArgs da = a;
+            da.idx = cand;
+            da.idx_count_from_ctrl = 1;
+            da.persistent = 1;
+            da.mode = topk ? kModeCollect : kModeBest;
+            rc = run_algo(e, algo, da, wc, metric);
+        }
+    } else {
+        // LB_Kim needs M >= 11 (bounds.cuh:418): evaluate every window.
+        BandArgs da = a;
+        da.count = L;
+        da.persistent = algo == MDTW_ALGO_ED ? 0 : 1;
+        da.mode = topk ? kModeCollect : kModeBest;
+        rc = run_algo(e, algo, da, wc, metric);
+    }
+    if (rc != MDTW_OK) return rc;
+    rc = hip_check(hipEventRecord(e->ev1, st));
+    if (rc == MDTW_OK)
+        rc = hip_check(hipMemcpyAsync(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    if (rc == MDTW_OK) rc = hip_check(hipStreamSynchronize(st));
+    if (rc != MDTW_OK) return rc;
+    const Ctrl c = *e->h_ctrl;
+
+    int32_t found = 0;
+    float thr_out;
+    if (!topk) {
+        if ((c.best >> 32) < 0x7f800000ull) {
+            const uint32_t bits = static_cast<uint32_t>(c.best >> 32);
+            float d;
+            std::memcpy(&d, &bits, sizeof(d));
+            h_pos[0] = static_cast<int64_t>(c.best & 0xffffffffull);
+            h_dist[0] = metric == MDTW_METRIC_L2 ? std::sqrt(d) : d;
+            found = 1;
+            thr_out = d;
+        } else {
+            thr_out = INFINITY;
+        }
+    } else {
+        thr_out = c.thr;
+        std::vector<unsigned long long> keys(static_cast<size_t>(c.nres));
+        if (c.nres) {
+            rc = hip_check(hipMemcpyAsync(keys.data(), res, sizeof(unsigned long long) * c.nres,
+                                          hipMemcpyDeviceToHost, st));
+            if (rc == MDTW_OK) rc = hip_check(hipStreamSynchronize(st));
+            if (rc != MDTW_OK) return rc;
+        }
+        std::sort(keys.begin(), keys.end());
+        for (unsigned long long key : keys) {
+            if (found == k) break;
+            const int64_t pos = static_cast<int64_t>(key & 0xffffffffull);
+            bool ok = true;
+            for (int32_t p = 0; p < found; ++p) {
+                int64_t dd = pos - h_pos[p];
+                if (dd < 0) dd = -dd;
+                if (dd < excl) {
+                    ok = false;
+                    break;
+                }
+            }
+            if (!ok) continue;
+            const uint32_t bits = static_cast<uint32_t>(key >> 32);
+            float d;
+            std::memcpy(&d, &bits, sizeof(d));
+            h_pos[found] = pos;
+            h_dist[found] = metric == MDTW_METRIC_L2 ? std::sqrt(d) : d;
+            found++;
+        }
+    }
+    if (info) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
+        info->windows = L;
+        info->seeded = sp.count;
+        info->lb_kim_pruned = static_cast<int64_t>(c.kim);
+        info->lb_keogh_pruned = static_cast<int64_t>(c.keogh);
+        info->dp_evaluated = static_cast<int64_t>(c.dp);
+        info->dp_abandoned = static_cast<int64_t>(c.abandoned);
+        info->found = found;
+        info->threshold = thr_out;
+        info->elapsed_ms = ms;
+    }
+    return MDTW_OK;
+}

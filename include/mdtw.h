@@ -1,0 +1,177 @@
+/*
+ * mdtw.h -- C ABI of the MI355X (gfx950) subsequence similarity-search
+ * engine.  Plain pointers and sizes only; every entry point returns an int
+ * status (MDTW_OK = 0) instead of exiting, and never prints.
+ *
+ * Each declaration names the reference interface it replaces
+ * (gravitino/cudadtw, file:line).  The reference is header-only C++ that
+ * keeps the query in __constant__ memory (cdtw.cuh:9-11) and prints its
+ * result (cdtw.cuh:493-508); here the query lives in an engine object and
+ * results are returned.
+ *
+ * Conventions
+ *   - d_* arguments are device pointers (HBM), h_* are host pointers.
+ *   - Stream data is FP32; window statistics are FP32 (mean, population
+ *     sigma), computed from FP64 prefix sums.
+ *   - Window i is series[i .. i+M); there are L = N - M + 1 windows.
+ *   - DP outputs are squared cDTW distances, as the reference's Cdtw array
+ *     (cdtw.cuh:256); reported search distances are sqrt of that
+ *     (cdtw.cuh:507).  Ties go to the lowest window index.
+ *   - Index arrays on the device are int32 (the reference's index_t = int);
+ *     host-facing positions are int64.
+ *   - One engine per host thread.  Calls on one engine are ordered on its
+ *     stream; distinct engines may run concurrently.
+ */
+#ifndef MDTW_H
+#define MDTW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define MDTW_OK                    0
+#define MDTW_ERR_INVALID_ARGUMENT  1  /* null pointer, negative size          */
+#define MDTW_ERR_QUERY_TOO_SHORT   2  /* M < 11 for LB_Kim (bounds.cuh:418)   */
+#define MDTW_ERR_QUERY_TOO_LONG    3  /* M > MDTW_MAX_QUERY                   */
+#define MDTW_ERR_BAND              4  /* W < 0 or W > M - 1                   */
+#define MDTW_ERR_STREAM_TOO_SHORT  5  /* N < M                                */
+#define MDTW_ERR_NO_QUERY          6  /* mdtw_set_query not called           */
+#define MDTW_ERR_ALLOC             7  /* device or host allocation failed     */
+#define MDTW_ERR_HIP               8  /* HIP runtime error                    */
+#define MDTW_ERR_NOT_FINITE        9  /* NaN/Inf in the query                 */
+#define MDTW_ERR_K                10  /* k < 1 or k > MDTW_MAX_K              */
+#define MDTW_ERR_UNSUPPORTED      11  /* option not available                 */
+#define MDTW_ERR_WORKSPACE        12  /* request exceeds the workspace cap    */
+
+#define MDTW_MAX_QUERY 8192   /* ed.cuh:10 MAXQUERYSIZE (cdtw.cuh:8 has 4096) */
+#define MDTW_MAX_K     1024
+
+/* metric (the reference's template<bool euclidean>, GPU-dtw.cu:7-8) */
+#define MDTW_METRIC_L1 0
+#define MDTW_METRIC_L2 1
+
+/* window statistics mode */
+#define MDTW_STATS_FP64      0  /* tile-local FP64 prefix sums, FP64 finalize */
+#define MDTW_STATS_REFCOMPAT 1  /* stats.cuh:85-142: FP64 epoch prefix, FP32
+                                   finalize fmaf(-X, X, X2), sigma := 1 if <= 0 */
+
+/* search algorithm */
+#define MDTW_ALGO_CDTW 0  /* cdtw.cuh:510-592 prune_cdtw            */
+#define MDTW_ALGO_ED   1  /* ed.cuh:71-153 calculate_ed            */
+#define MDTW_ALGO_LDTW 2  /* ldtw.cuh:294-391 ztsh2_ldtw + argmin  */
+
+typedef struct mdtw_engine mdtw_engine;
+
+/* What a search did, stage by stage (UCR_DTW_BIN.cpp:738-741 prints the same
+ * prune fractions; cdtw.cuh:576 has the commented-out survival print). */
+typedef struct mdtw_search_info {
+    int64_t windows;        /* L = N - M + 1                                 */
+    int64_t seeded;         /* windows evaluated exactly to seed the bound   */
+    int64_t lb_kim_pruned;  /* windows dropped by LB_Kim                     */
+    int64_t lb_keogh_pruned;/* windows dropped by LB_Keogh                   */
+    int64_t dp_evaluated;   /* windows whose DP (or LDTW/ED) ran             */
+    int64_t dp_abandoned;   /* DP runs stopped early by the row-minimum test */
+    int32_t found;          /* matches returned (<= k)                       */
+    float   threshold;      /* final pruning threshold (squared distance)    */
+    float   elapsed_ms;     /* device time of the whole search (HIP events)  */
+} mdtw_search_info;
+
+/* ---- engine ------------------------------------------------------------- */
+/* Replaces the reference's process-global state: cudaSetDevice (GPU-dtw.cu:19)
+ * and the __constant__ query/envelope and texture (cdtw.cuh:9-14). */
+int mdtw_create(int device, mdtw_engine** out);
+int mdtw_destroy(mdtw_engine* e);
+/* Every launch of the engine goes to this stream (hipStream_t; NULL = the
+ * default stream).  The reference takes cudaStream_t per call and ignores it
+ * in places (stats.cuh:104-108, ed.cuh:111). */
+int mdtw_set_stream(mdtw_engine* e, void* hip_stream);
+int mdtw_synchronize(mdtw_engine* e);
+const char* mdtw_status_string(int status);
+/* Device name, compute-unit count; for logs. */
+int mdtw_device_info(mdtw_engine* e, char* name, int32_t name_len, int32_t* cus);
+
+/* ---- query -------------------------------------------------------------- */
+/* Replaces cudaMemcpyToSymbol(Czquery/Czlower/Czupper) (GPU-dtw.cu:82-84).
+ * h_zquery is the z-normalised query (mdtw_znormalize); h_lower/h_upper are
+ * its envelope for band w (mdtw_envelope), or NULL to have the engine build
+ * it for band w.  The query is copied; the host arrays may be reused. */
+int mdtw_set_query(mdtw_engine* e, const float* h_zquery, const float* h_lower,
+                   const float* h_upper, int32_t m, int32_t w);
+
+/* ---- host helpers (bit-exact restatements) ------------------------------ */
+/* stats.cuh:8-28 znormalize<float>: FP32 accumulation in index order. */
+int mdtw_znormalize(float* h_series, int32_t len);
+/* bounds.cuh:13-64 envelope (Lemire). */
+int mdtw_envelope(const float* h_series, int32_t w, float* h_lower,
+                  float* h_upper, int32_t m);
+
+/* ---- window statistics -------------------------------------------------- */
+/* stats.cuh:115-142 avg_std<double>(Series, AvgS, StdS, M, N, stream). */
+int mdtw_avg_std(mdtw_engine* e, const float* d_series, int64_t n, int32_t m,
+                 float* d_avg, float* d_std, int32_t mode);
+
+/* ---- lower bounds (need mdtw_set_query) ---------------------------------- */
+/* bounds.cuh:413-441 lb_kim: d_indices NULL = all count windows 0..count-1
+ * (register_lb_kim), else the gathered windows (random_lb_kim).  M >= 11. */
+int mdtw_lb_kim(mdtw_engine* e, const float* d_series, const float* d_avg,
+                const float* d_std, float* d_out, const int32_t* d_indices,
+                int64_t count, int32_t metric);
+/* bounds.cuh:443-465 lb_keogh against the query envelope. */
+int mdtw_lb_keogh(mdtw_engine* e, const float* d_series, const float* d_avg,
+                  const float* d_std, float* d_out, const int32_t* d_indices,
+                  int64_t count, int32_t metric);
+
+/* ---- distance profiles (need mdtw_set_query) ----------------------------- */
+/* cdtw.cuh:450-487 gpu_cdtw: squared banded cDTW of the query against the
+ * windows d_indices[0..count) (NULL = windows 0..count-1), band w. */
+int mdtw_cdtw(mdtw_engine* e, const float* d_series, const float* d_avg,
+              const float* d_std, float* d_out, const int32_t* d_indices,
+              int64_t count, int32_t w, int32_t metric);
+/* ldtw.cuh:294-391 ztsh2_ldtw: greedy LDTW of every window 0..count-1. */
+int mdtw_ldtw(mdtw_engine* e, const float* d_series, const float* d_avg,
+              const float* d_std, float* d_out, int64_t count, int32_t w,
+              int32_t metric);
+/* ed.cuh:61-69 finalize: squared z-normalised ED of windows 0..count-1
+ * (the reference stores 2M-shifted values, ed.cuh:68; this stores ED^2). */
+int mdtw_ed(mdtw_engine* e, const float* d_series, const float* d_avg,
+            const float* d_std, float* d_out, int64_t count);
+
+/* No reference counterpart: the same profile as mdtw_cdtw, forced through the
+ * anti-diagonal kernel that serves bands wider than the register-band kernel
+ * (for cross-checking the two kernels against each other). */
+int mdtw_cdtw_wide(mdtw_engine* e, const float* d_series, const float* d_avg,
+                   const float* d_std, float* d_out, const int32_t* d_indices,
+                   int64_t count, int32_t w, int32_t metric);
+
+/* ---- search -------------------------------------------------------------- */
+/* cdtw.cuh:510-592 prune_cdtw / ed.cuh:71-153 calculate_ed / GPU-ldtw.cu:
+ * best (k = 1) or top-k matches of the engine's query in d_series[0..n).
+ * d_avg/d_std: window statistics for M (NULL = computed here in stats_mode).
+ * excl: exclusion zone for k > 1 (a match excludes windows j with
+ * |i - j| < excl); ignored for k = 1.  Outputs are host arrays of k entries:
+ * positions (ascending distance) and distances (sqrt for L2, as
+ * cdtw.cuh:507).  Unused entries get position -1.  info may be NULL. */
+int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n,
+                const float* d_avg, const float* d_std, int32_t w,
+                int32_t algo, int32_t metric, int32_t stats_mode, int32_t k,
+                int64_t excl, int64_t* h_pos, float* h_dist,
+                mdtw_search_info* info);
+
+/* ---- primitives (cub_util.cuh) ------------------------------------------- */
+/* cub_util.cuh:154-172 threshold: keeps d_indices[i] with d_values[i] <= thr,
+ * in order, in place; the new length goes to *h_count. */
+int mdtw_threshold(mdtw_engine* e, const float* d_values, int32_t* d_indices,
+                   int64_t count, float thr, int64_t* h_count);
+/* cub_util.cuh:7-27 reduce(Min) + report_result (cdtw.cuh:493-508): lowest
+ * value and its lowest index among d_values[0..count). */
+int mdtw_argmin(mdtw_engine* e, const float* d_values, int64_t count,
+                int64_t* h_index, float* h_value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MDTW_H */

@@ -1,0 +1,850 @@
+/*
+ * oracle.c -- CPU restatement of the reference's subsequence-search path.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): the checker, never the product.
+ *
+ * Build: gcc -O2 -ffp-contract=off -mfma -fopenmp (oracle/Makefile).  All
+ * fused multiply-adds below are explicit; nothing else may be contracted.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_INF ((float)INFINITY)
+
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline float fmin2(float a, float b) { return a < b ? a : b; }
+
+/* Local cost of one cell, lp(euclidean, d) in bounds.cuh:150. */
+static inline float lp(float d, int l2) { return l2 ? d * d : fabsf(d); }
+/* lp(euclidean, d) + acc as nvcc contracts it (one rounding for L2). */
+static inline float lpa(float d, float acc, int l2) {
+    return l2 ? fmaf(d, d, acc) : fabsf(d) + acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* stats.cuh:8-28 znormalize: mean, then sqrt(E[x^2] - mean^2), both
+ * accumulated in value_t in index order; then (x - avg) / std in place. */
+void or_znormalize_f32(float* x, int32_t n) {
+    float avg = 0.0f, sd = 0.0f;
+    for (int32_t i = 0; i < n; ++i) avg += x[i];
+    avg /= (float)n;
+    for (int32_t i = 0; i < n; ++i) sd += x[i] * x[i];
+    sd = sqrtf(sd / (float)n - avg * avg);
+    for (int32_t i = 0; i < n; ++i) {
+        x[i] -= avg;
+        x[i] /= sd;
+    }
+}
+
+void or_znormalize_f64(double* x, int32_t n) {
+    double avg = 0.0, sd = 0.0;
+    for (int32_t i = 0; i < n; ++i) avg += x[i];
+    avg /= (double)n;
+    for (int32_t i = 0; i < n; ++i) sd += x[i] * x[i];
+    sd = sqrt(sd / (double)n - avg * avg);
+    for (int32_t i = 0; i < n; ++i) {
+        x[i] -= avg;
+        x[i] /= sd;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* bounds.cuh:13-64: Lemire's streaming max/min with two index deques.  The
+ * deques only ever grow at the back, so a flat array with head/tail indices
+ * holds them.  w >= m is clamped to m - 1 (the envelope is then the global
+ * max/min; the reference indexes out of bounds there). */
+void or_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m) {
+    if (m <= 0) return;
+    if (w > m - 1) w = m - 1;
+    if (w < 0) w = 0;
+    int* ub = (int*)malloc(sizeof(int) * (size_t)(m + 1));
+    int* lb = (int*)malloc(sizeof(int) * (size_t)(m + 1));
+    int uh = 0, ut = 0, lh = 0, lt = 0;
+    ub[ut++] = 0;
+    lb[lt++] = 0;
+    for (int i = 1; i < m; ++i) {
+        if (i > w) {
+            up[i - w - 1] = q[ub[uh]];
+            lo[i - w - 1] = q[lb[lh]];
+        }
+        if (q[i] > q[i - 1]) {
+            ut--;
+            while (ut > uh && q[i] > q[ub[ut - 1]]) ut--;
+        } else {
+            lt--;
+            while (lt > lh && q[i] < q[lb[lt - 1]]) lt--;
+        }
+        ub[ut++] = i;
+        lb[lt++] = i;
+        if (i == 2 * w + 1 + ub[uh])
+            uh++;
+        else if (i == 2 * w + 1 + lb[lh])
+            lh++;
+    }
+    for (int i = m; i < m + w + 1; ++i) {
+        up[i - w - 1] = q[ub[uh]];
+        lo[i - w - 1] = q[lb[lh]];
+        if (i - ub[uh] >= 2 * w + 1) uh++;
+        if (i - lb[lh] >= 2 * w + 1) lh++;
+    }
+    free(ub);
+    free(lb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* stats.cuh:85-113 for one epoch: FP64 inclusive prefix with P[0] = 0
+ * (cub_util.cuh:29-54), FP64 window difference / M stored as FP32
+ * (stats.cuh:30-38), FP32 finalize (stats.cuh:40-52). */
+static void refcompat_epoch(const float* x, float* avg, float* sd, int32_t m,
+                            int64_t len) {
+    double* p1 = (double*)malloc(sizeof(double) * (size_t)(len + 1));
+    double* p2 = (double*)malloc(sizeof(double) * (size_t)(len + 1));
+    p1[0] = 0.0;
+    p2[0] = 0.0;
+    for (int64_t k = 0; k < len; ++k) {
+        double v = (double)x[k];
+        p1[k + 1] = p1[k] + v;
+        p2[k + 1] = p2[k] + v * v;
+    }
+    for (int64_t i = 0; i + m <= len; ++i) {
+        float X = (float)((p1[i + m] - p1[i]) / (double)m);
+        float X2 = (float)((p2[i + m] - p2[i]) / (double)m);
+        float v = fmaf(-X, X, X2);
+        avg[i] = X;
+        sd[i] = v > 0.0f ? sqrtf(v) : 1.0f;
+    }
+    free(p1);
+    free(p2);
+}
+
+/* stats.cuh:115-142 avg_std: epochs of `epoch` samples advancing by
+ * epoch - M + 1; a final chunk of at most M + 1 samples is merged into the
+ * previous epoch.  (With a single short chunk the reference reads an empty
+ * vector; here that chunk is simply processed.) */
+void or_stats_refcompat(const float* s, int64_t n, int32_t m, int64_t epoch,
+                        float* avg, float* sd) {
+    if (epoch <= 0) epoch = 10000000;
+    int64_t width = epoch - m + 1;
+    if (width < 1) width = 1;
+    int64_t cap = n / width + 2;
+    int64_t* lowers = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+    int64_t* uppers = (int64_t*)malloc(sizeof(int64_t) * (size_t)cap);
+    int64_t cnt = 0;
+    for (int64_t lower = 0; lower < n; lower += width) {
+        int64_t upper = lower + epoch < n ? lower + epoch : n;
+        if (upper - lower > m + 1 || cnt == 0) {
+            lowers[cnt] = lower;
+            uppers[cnt] = upper;
+            cnt++;
+        } else {
+            uppers[cnt - 1] = upper;
+        }
+    }
+    for (int64_t e = 0; e < cnt; ++e)
+        refcompat_epoch(s + lowers[e], avg + lowers[e], sd + lowers[e], m,
+                        uppers[e] - lowers[e]);
+    free(lowers);
+    free(uppers);
+}
+
+void or_stats_fp64(const float* s, int64_t n, int32_t m, float* avg, float* sd,
+                   double* avg64, double* sd64) {
+    int64_t L = n - m + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < L; ++i) {
+        double sum = 0.0;
+        for (int32_t k = 0; k < m; ++k) sum += (double)s[i + k];
+        double mean = sum / (double)m;
+        double ss = 0.0;
+        for (int32_t k = 0; k < m; ++k) {
+            double d = (double)s[i + k] - mean;
+            ss += d * d;
+        }
+        double var = ss / (double)m;
+        double sdev = var > 0.0 ? sqrt(var) : 1.0;
+        avg[i] = (float)mean;
+        sd[i] = (float)sdev;
+        if (avg64) avg64[i] = mean;
+        if (sd64) sd64[i] = sdev;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* bounds.cuh:283-407 register_lb_kim / :152-281 random_lb_kim. */
+static float lb_kim_one(const float* s, float avg, float sdv, const float* zq,
+                        int32_t m, int l2) {
+    float q0, q1, q2, q3, q4, s0, s1, s2, s3, s4, mem;
+    float p0, p1, p2, p3, p4, p5, p6, p7, p8, p9;
+    q0 = zq[0]; q1 = zq[1]; q2 = zq[2]; q3 = zq[3]; q4 = zq[4];
+    s0 = (s[0] - avg) / sdv;
+    s1 = (s[1] - avg) / sdv;
+    s2 = (s[2] - avg) / sdv;
+    s3 = (s[3] - avg) / sdv;
+    s4 = (s[4] - avg) / sdv;
+
+    p0 = lp(q0 - s0, l2);
+    p1 = lpa(q0 - s1, p0, l2);
+    p2 = lpa(q0 - s2, p1, l2);
+    p3 = lpa(q0 - s3, p2, l2);
+    p4 = lpa(q0 - s4, p3, l2);
+    mem = p4;
+
+    p5 = lpa(q1 - s0, p0, l2);
+    p6 = lpa(q1 - s1, fmin2(p5, fmin2(p0, p1)), l2);
+    p7 = lpa(q1 - s2, fmin2(p6, fmin2(p1, p2)), l2);
+    p8 = lpa(q1 - s3, fmin2(p7, fmin2(p2, p3)), l2);
+    p9 = lpa(q1 - s4, fmin2(p8, fmin2(p3, p4)), l2);
+    mem = fmin2(mem, p9);
+
+    p0 = lpa(q2 - s0, p5, l2);
+    p1 = lpa(q2 - s1, fmin2(p0, fmin2(p5, p6)), l2);
+    p2 = lpa(q2 - s2, fmin2(p1, fmin2(p6, p7)), l2);
+    p3 = lpa(q2 - s3, fmin2(p2, fmin2(p7, p8)), l2);
+    p4 = lpa(q2 - s4, fmin2(p3, fmin2(p8, p9)), l2);
+    mem = fmin2(mem, p4);
+
+    p5 = lpa(q3 - s0, p0, l2);
+    p6 = lpa(q3 - s1, fmin2(p5, fmin2(p0, p1)), l2);
+    p7 = lpa(q3 - s2, fmin2(p6, fmin2(p1, p2)), l2);
+    p8 = lpa(q3 - s3, fmin2(p7, fmin2(p2, p3)), l2);
+    p9 = lpa(q3 - s4, fmin2(p8, fmin2(p3, p4)), l2);
+    mem = fmin2(mem, p9);
+
+    p0 = lpa(q4 - s0, p5, l2);
+    p1 = lpa(q4 - s1, fmin2(p0, fmin2(p5, p6)), l2);
+    p2 = lpa(q4 - s2, fmin2(p1, fmin2(p6, p7)), l2);
+    p3 = lpa(q4 - s3, fmin2(p2, fmin2(p7, p8)), l2);
+    p4 = lpa(q4 - s4, fmin2(p3, fmin2(p8, p9)), l2);
+    mem = fmin2(mem, fmin2(p0, fmin2(p1, fmin2(p2, fmin2(p3, p4)))));
+
+    q0 = zq[m - 5]; q1 = zq[m - 4]; q2 = zq[m - 3]; q3 = zq[m - 2]; q4 = zq[m - 1];
+    s0 = (s[m - 5] - avg) / sdv;
+    s1 = (s[m - 4] - avg) / sdv;
+    s2 = (s[m - 3] - avg) / sdv;
+    s3 = (s[m - 2] - avg) / sdv;
+    s4 = (s[m - 1] - avg) / sdv;
+
+    p0 = lp(q0 - s0, l2);
+    p1 = lp(q0 - s1, l2);
+    p2 = lp(q0 - s2, l2);
+    p3 = lp(q0 - s3, l2);
+    p4 = lp(q0 - s4, l2);
+
+    p5 = lp(q1 - s0, l2);
+    p6 = lpa(q1 - s1, fmin2(p5, fmin2(p0, p1)), l2);
+    p7 = lpa(q1 - s2, fmin2(p6, fmin2(p1, p2)), l2);
+    p8 = lpa(q1 - s3, fmin2(p7, fmin2(p2, p3)), l2);
+    p9 = lpa(q1 - s4, fmin2(p8, fmin2(p3, p4)), l2);
+
+    p0 = lp(q2 - s0, l2);
+    p1 = lpa(q2 - s1, fmin2(p0, fmin2(p5, p6)), l2);
+    p2 = lpa(q2 - s2, fmin2(p1, fmin2(p6, p7)), l2);
+    p3 = lpa(q2 - s3, fmin2(p2, fmin2(p7, p8)), l2);
+    p4 = lpa(q2 - s4, fmin2(p3, fmin2(p8, p9)), l2);
+
+    p5 = lp(q3 - s0, l2);
+    p6 = lpa(q3 - s1, fmin2(p5, fmin2(p0, p1)), l2);
+    p7 = lpa(q3 - s2, fmin2(p6, fmin2(p1, p2)), l2);
+    p8 = lpa(q3 - s3, fmin2(p7, fmin2(p2, p3)), l2);
+    p9 = lpa(q3 - s4, fmin2(p8, fmin2(p3, p4)), l2);
+
+    p0 = lp(q4 - s0, l2);
+    p1 = lpa(q4 - s1, fmin2(p0, fmin2(p5, p6)), l2);
+    p2 = lpa(q4 - s2, fmin2(p1, fmin2(p6, p7)), l2);
+    p3 = lpa(q4 - s3, fmin2(p2, fmin2(p7, p8)), l2);
+    p4 = lpa(q4 - s4, fmin2(p3, fmin2(p8, p9)), l2);
+
+    return mem + p4;
+}
+
+void or_lb_kim(const float* s, const float* avg, const float* sd,
+               const float* zq, int32_t m, const int32_t* idx, int64_t count,
+               int32_t metric, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < count; ++g) {
+        int64_t w = idx ? idx[g] : g;
+        out[g] = lb_kim_one(s + w, avg[w], sd[w], zq, m, metric);
+    }
+}
+
+/* bounds.cuh:111-144.  The reference adds u*u*(u>0) + l*l*(l<0); at most one
+ * term is non-zero (lo <= up), so the sum is accumulated with one fmaf. */
+void or_lb_keogh(const float* s, const float* avg, const float* sd,
+                 const float* lo, const float* up, int32_t m,
+                 const int32_t* idx, int64_t count, int32_t metric, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < count; ++g) {
+        int64_t w = idx ? idx[g] : g;
+        float a = avg[w], sdv = sd[w], acc = 0.0f;
+        for (int32_t i = 0; i < m; ++i) {
+            float v = (s[w + i] - a) / sdv;
+            float lower = v - lo[i];
+            float upper = v - up[i];
+            if (metric) {
+                if (upper > 0.0f)
+                    acc = fmaf(upper, upper, acc);
+                else if (lower < 0.0f)
+                    acc = fmaf(lower, lower, acc);
+            } else {
+                if (upper > 0.0f)
+                    acc = acc + upper;
+                else if (lower < 0.0f)
+                    acc = acc - lower;
+            }
+        }
+        out[g] = acc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* cdtw.cuh:23-88 sequential_cdtw: two rows of M+1 cells, row i = query
+ * index, band |i - j| <= W, halo cells set to INFINITY. */
+void or_cdtw(const float* zq, const float* s, const float* avg, const float* sd,
+             int32_t m, const int32_t* idx, int64_t count, int32_t w,
+             int32_t metric, int32_t fused, float* out) {
+    const int lane = m + 1;
+#pragma omp parallel
+    {
+        float* pen = (float*)malloc(sizeof(float) * 2 * (size_t)lane);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t g = 0; g < count; ++g) {
+            int64_t ix = idx ? idx[g] : g;
+            const float* sub = s + ix;
+            float a = avg[ix], sdv = sd[ix];
+            for (int k = 0; k < 2 * lane; ++k) pen[k] = ORC_INF;
+            pen[0] = 0.0f;
+            int src = 1, trg = 0;
+            for (int i = 1; i < lane; ++i) {
+                trg = src;
+                src = (trg == 0);
+                int lower = imax(0, i - w - 1), upper = imin(lane, i + w + 1);
+                pen[trg * lane + lower] = ORC_INF;
+                for (int j = lower + 1; j < upper; ++j) {
+                    float value = zq[i - 1] - (sub[j - 1] - a) / sdv;
+                    float mn = fmin2(pen[src * lane + j - 1],
+                                     fmin2(pen[src * lane + j], pen[trg * lane + j - 1]));
+                    float wt;
+                    if (metric)
+                        wt = fused ? fmaf(value, value, mn) : value * value + mn;
+                    else
+                        wt = fabsf(value) + mn;
+                    pen[trg * lane + j] = wt;
+                }
+                if (upper < lane) pen[trg * lane + upper] = ORC_INF;
+            }
+            out[g] = pen[trg * lane + m];
+        }
+        free(pen);
+    }
+}
+
+void or_cdtw_f64(const double* zq, const float* s, const double* avg,
+                 const double* sd, int32_t m, const int32_t* idx, int64_t count,
+                 int32_t w, int32_t metric, double* out) {
+    const int lane = m + 1;
+#pragma omp parallel
+    {
+        double* pen = (double*)malloc(sizeof(double) * 2 * (size_t)lane);
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t g = 0; g < count; ++g) {
+            int64_t ix = idx ? idx[g] : g;
+            const float* sub = s + ix;
+            double a = avg[ix], sdv = sd[ix];
+            for (int k = 0; k < 2 * lane; ++k) pen[k] = INFINITY;
+            pen[0] = 0.0;
+            int src = 1, trg = 0;
+            for (int i = 1; i < lane; ++i) {
+                trg = src;
+                src = (trg == 0);
+                int lower = imax(0, i - w - 1), upper = imin(lane, i + w + 1);
+                pen[trg * lane + lower] = INFINITY;
+                for (int j = lower + 1; j < upper; ++j) {
+                    double value = zq[i - 1] - ((double)sub[j - 1] - a) / sdv;
+                    double mn = pen[src * lane + j - 1];
+                    if (pen[src * lane + j] < mn) mn = pen[src * lane + j];
+                    if (pen[trg * lane + j - 1] < mn) mn = pen[trg * lane + j - 1];
+                    pen[trg * lane + j] = (metric ? value * value : fabs(value)) + mn;
+                }
+                if (upper < lane) pen[trg * lane + upper] = INFINITY;
+            }
+            out[g] = pen[trg * lane + m];
+        }
+        free(pen);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* ldtw.cuh:74-127 z_serial_ldtw; INFTY = 1E10 (ldtw.cuh:8). */
+#define LDTW_INFTY ((float)1E10)
+
+static float ldtw_one(const float* q, const float* sub, float a, float sdv,
+                      int32_t M, int32_t W, int l2, uint8_t* moves) {
+    int i = 0, j = 0, step = 0;
+    float d = q[i] - (sub[j] - a) / sdv;
+    d = l2 ? d * d : (d < 0 ? -d : d);
+    while (i + 1 != M || j + 1 != M) {
+        float diag = LDTW_INFTY, rght = LDTW_INFTY, down = LDTW_INFTY;
+        if (i + 1 < M && j + 1 < M) {
+            diag = q[i + 1] - (sub[j + 1] - a) / sdv;
+            diag = l2 ? diag * diag : (diag < 0 ? -diag : diag);
+        }
+        if (i + 1 < M && abs(i - j + 1) <= W) {
+            down = q[i + 1] - (sub[j] - a) / sdv;
+            down = l2 ? down * down : (down < 0 ? -down : down);
+        }
+        if (j + 1 < M && abs(j - i + 1) <= W) {
+            rght = q[i] - (sub[j + 1] - a) / sdv;
+            rght = l2 ? rght * rght : (rght < 0 ? -rght : rght);
+        }
+        const int diagSdown = diag < down;
+        const int diagSrght = diag < rght;
+        const int downSrght = down < rght;
+        uint8_t mv;
+        if (diagSdown && diagSrght) {
+            d += diag;
+            i++;
+            j++;
+            mv = 0;
+        } else if (!diagSdown && downSrght) {
+            d += down;
+            i++;
+            mv = 1;
+        } else {
+            d += rght;
+            j++;
+            mv = 2;
+        }
+        if (moves) moves[step] = mv;
+        step++;
+    }
+    if (moves)
+        for (; step < 2 * M - 2; ++step) moves[step] = 3;
+    return d;
+}
+
+void or_ldtw(const float* zq, const float* s, const float* avg, const float* sd,
+             int32_t m, const int32_t* idx, int64_t count, int32_t w,
+             int32_t metric, float* out, uint8_t* moves) {
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < count; ++g) {
+        int64_t ix = idx ? idx[g] : g;
+        out[g] = ldtw_one(zq, s + ix, avg[ix], sd[ix], m, w, metric,
+                          moves ? moves + g * (int64_t)(2 * m - 2) : NULL);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+void or_ed_f64(const double* zq, const float* s, int64_t n, int32_t m,
+               const double* avg64, const double* sd64, double* out) {
+    int64_t L = n - m + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < L; ++i) {
+        double acc = 0.0, a = avg64[i], sdv = sd64[i];
+        for (int32_t k = 0; k < m; ++k) {
+            double z = ((double)s[i + k] - a) / sdv;
+            double d = zq[k] - z;
+            acc += d * d;
+        }
+        out[i] = acc;
+    }
+}
+
+int64_t or_argmin(const float* x, int64_t n) {
+    float bsf = ORC_INF;
+    int64_t bi = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (x[i] < bsf) {
+            bsf = x[i];
+            bi = i;
+        }
+    return bi;
+}
+
+/* ------------------------------------------------------------------------ */
+static const float* g_sort_d;
+static int cmp_dist_idx(const void* a, const void* b) {
+    int64_t i = *(const int64_t*)a, j = *(const int64_t*)b;
+    float di = g_sort_d[i], dj = g_sort_d[j];
+    if (di < dj) return -1;
+    if (di > dj) return 1;
+    return i < j ? -1 : (i > j ? 1 : 0);
+}
+
+int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
+                       int64_t* out_idx, float* out_dist) {
+    int64_t* ord = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) ord[i] = i;
+    g_sort_d = d;
+    qsort(ord, (size_t)n, sizeof(int64_t), cmp_dist_idx);
+    int32_t got = 0;
+    for (int64_t t = 0; t < n && got < k; ++t) {
+        int64_t c = ord[t];
+        int ok = 1;
+        for (int32_t p = 0; p < got; ++p) {
+            int64_t dd = c - out_idx[p];
+            if (dd < 0) dd = -dd;
+            if (dd < excl) {
+                ok = 0;
+                break;
+            }
+        }
+        if (ok) {
+            out_idx[got] = c;
+            out_dist[got] = d[c];
+            got++;
+        }
+    }
+    free(ord);
+    return got;
+}
+
+/* ------------------------------------------------------------------------ */
+/* UCR Suite port (UCR_DTW_BIN.cpp).  FP64 throughout, as the original. */
+#define UCR_INF 1e20
+#define UCR_DIST(x, y) (((x) - (y)) * ((x) - (y)))
+
+/* UCR_DTW_BIN.cpp:129-176 lower_upper_lemire (same algorithm as
+ * bounds.cuh:13-64, on doubles, with the circular deque replaced by a flat
+ * one). */
+static void ucr_lemire(const double* t, int len, int r, double* l, double* u) {
+    int* du = (int*)malloc(sizeof(int) * (size_t)(len + 1));
+    int* dl = (int*)malloc(sizeof(int) * (size_t)(len + 1));
+    int uh = 0, ut = 0, lh = 0, lt = 0;
+    du[ut++] = 0;
+    dl[lt++] = 0;
+    for (int i = 1; i < len; i++) {
+        if (i > r) {
+            u[i - r - 1] = t[du[uh]];
+            l[i - r - 1] = t[dl[lh]];
+        }
+        if (t[i] > t[i - 1]) {
+            ut--;
+            while (ut > uh && t[i] > t[du[ut - 1]]) ut--;
+        } else {
+            lt--;
+            while (lt > lh && t[i] < t[dl[lt - 1]]) lt--;
+        }
+        du[ut++] = i;
+        dl[lt++] = i;
+        if (i == 2 * r + 1 + du[uh])
+            uh++;
+        else if (i == 2 * r + 1 + dl[lh])
+            lh++;
+    }
+    for (int i = len; i < len + r + 1; i++) {
+        if (i - r - 1 >= 0) {
+            u[i - r - 1] = t[du[uh]];
+            l[i - r - 1] = t[dl[lh]];
+        }
+        if (i - du[uh] >= 2 * r + 1) uh++;
+        if (i - dl[lh] >= 2 * r + 1) lh++;
+    }
+    free(du);
+    free(dl);
+}
+
+/* UCR_DTW_BIN.cpp:183-224 */
+static double ucr_lb_kim_hierarchy(const double* t, const double* q, int j, int len,
+                                   double mean, double sdv, double bsf) {
+    double d, lb;
+    double x0 = (t[j] - mean) / sdv;
+    double y0 = (t[(len - 1 + j)] - mean) / sdv;
+    lb = UCR_DIST(x0, q[0]) + UCR_DIST(y0, q[len - 1]);
+    if (lb >= bsf) return lb;
+    double x1 = (t[(j + 1)] - mean) / sdv;
+    d = fmin(UCR_DIST(x1, q[0]), UCR_DIST(x0, q[1]));
+    d = fmin(d, UCR_DIST(x1, q[1]));
+    lb += d;
+    if (lb >= bsf) return lb;
+    double y1 = (t[(len - 2 + j)] - mean) / sdv;
+    d = fmin(UCR_DIST(y1, q[len - 1]), UCR_DIST(y0, q[len - 2]));
+    d = fmin(d, UCR_DIST(y1, q[len - 2]));
+    lb += d;
+    if (lb >= bsf) return lb;
+    double x2 = (t[(j + 2)] - mean) / sdv;
+    d = fmin(UCR_DIST(x0, q[2]), UCR_DIST(x1, q[2]));
+    d = fmin(d, UCR_DIST(x2, q[2]));
+    d = fmin(d, UCR_DIST(x2, q[1]));
+    d = fmin(d, UCR_DIST(x2, q[0]));
+    lb += d;
+    if (lb >= bsf) return lb;
+    double y2 = (t[(len - 3 + j)] - mean) / sdv;
+    d = fmin(UCR_DIST(y0, q[len - 3]), UCR_DIST(y1, q[len - 3]));
+    d = fmin(d, UCR_DIST(y2, q[len - 3]));
+    d = fmin(d, UCR_DIST(y2, q[len - 2]));
+    d = fmin(d, UCR_DIST(y2, q[len - 1]));
+    lb += d;
+    return lb;
+}
+
+/* UCR_DTW_BIN.cpp:235-252 */
+static double ucr_lb_keogh_cumulative(const int* order, const double* t,
+                                      const double* uo, const double* lo, double* cb,
+                                      int j, int len, double mean, double sdv,
+                                      double bsf) {
+    double lb = 0, x, d;
+    for (int i = 0; i < len && lb < bsf; i++) {
+        x = (t[(order[i] + j)] - mean) / sdv;
+        d = 0;
+        if (x > uo[i])
+            d = UCR_DIST(x, uo[i]);
+        else if (x < lo[i])
+            d = UCR_DIST(x, lo[i]);
+        lb += d;
+        cb[order[i]] = d;
+    }
+    return lb;
+}
+
+/* UCR_DTW_BIN.cpp:262-282 */
+static double ucr_lb_keogh_data_cumulative(const int* order, const double* qo,
+                                           double* cb, const double* l, const double* u,
+                                           int len, double mean, double sdv,
+                                           double bsf) {
+    double lb = 0, uu, ll, d;
+    for (int i = 0; i < len && lb < bsf; i++) {
+        uu = (u[order[i]] - mean) / sdv;
+        ll = (l[order[i]] - mean) / sdv;
+        d = 0;
+        if (qo[i] > uu)
+            d = UCR_DIST(qo[i], uu);
+        else if (qo[i] < ll)
+            d = UCR_DIST(qo[i], ll);
+        lb += d;
+        cb[order[i]] = d;
+    }
+    return lb;
+}
+
+/* UCR_DTW_BIN.cpp:288-354, with the two O(r) rows preallocated. */
+static double ucr_dtw(const double* A, const double* B, const double* cb, int m,
+                      int r, double bsf, double* cost, double* cost_prev) {
+    int i, j, k = 0;
+    double x, y, z, min_cost;
+    for (k = 0; k < 2 * r + 1; k++) cost[k] = UCR_INF;
+    for (k = 0; k < 2 * r + 1; k++) cost_prev[k] = UCR_INF;
+    for (i = 0; i < m; i++) {
+        k = imax(0, r - i);
+        min_cost = UCR_INF;
+        for (j = imax(0, i - r); j <= imin(m - 1, i + r); j++, k++) {
+            if ((i == 0) && (j == 0)) {
+                cost[k] = UCR_DIST(A[0], B[0]);
+                min_cost = cost[k];
+                continue;
+            }
+            if ((j - 1 < 0) || (k - 1 < 0))
+                y = UCR_INF;
+            else
+                y = cost[k - 1];
+            if ((i - 1 < 0) || (k + 1 > 2 * r))
+                x = UCR_INF;
+            else
+                x = cost_prev[k + 1];
+            if ((i - 1 < 0) || (j - 1 < 0))
+                z = UCR_INF;
+            else
+                z = cost_prev[k];
+            cost[k] = fmin(fmin(x, y), z) + UCR_DIST(A[i], B[j]);
+            if (cost[k] < min_cost) min_cost = cost[k];
+        }
+        if (i + r < m - 1 && min_cost + cb[i + r + 1] >= bsf)
+            return min_cost + cb[i + r + 1];
+        double* tmp = cost;
+        cost = cost_prev;
+        cost_prev = tmp;
+    }
+    k--;
+    return cost_prev[k];
+}
+
+typedef struct {
+    double v;
+    int i;
+} ucr_pair;
+
+static int ucr_cmp_abs_desc(const void* a, const void* b) {
+    const ucr_pair* x = (const ucr_pair*)a;
+    const ucr_pair* y = (const ucr_pair*)b;
+    double ax = fabs(x->v), ay = fabs(y->v);
+    if (ay > ax) return 1;
+    if (ay < ax) return -1;
+    return x->i - y->i;
+}
+
+/* UCR_DTW_BIN.cpp:527-707 main search loop. */
+int64_t or_ucr_dtw(const double* data, int64_t n, const double* query,
+                   int32_t m, int32_t r, double* dist, int64_t* counts) {
+    const int EPOCH = 100000;
+    double *q = malloc(sizeof(double) * m), *qo = malloc(sizeof(double) * m);
+    double *uo = malloc(sizeof(double) * m), *lo = malloc(sizeof(double) * m);
+    double *u = malloc(sizeof(double) * m), *l = malloc(sizeof(double) * m);
+    double *cb = calloc(m, sizeof(double)), *cb1 = calloc(m, sizeof(double));
+    double* cb2 = calloc(m, sizeof(double));
+    double *t = malloc(sizeof(double) * m * 2), *tz = malloc(sizeof(double) * m);
+    double* buffer = malloc(sizeof(double) * EPOCH);
+    double *u_buff = malloc(sizeof(double) * EPOCH), *l_buff = malloc(sizeof(double) * EPOCH);
+    double *cost = malloc(sizeof(double) * (2 * r + 1)), *cost_prev = malloc(sizeof(double) * (2 * r + 1));
+    int* order = malloc(sizeof(int) * m);
+    ucr_pair* qt = malloc(sizeof(ucr_pair) * m);
+
+    double ex = 0, ex2 = 0, mean, sdv, bsf = UCR_INF;
+    for (int i = 0; i < m; i++) {
+        ex += query[i];
+        ex2 += query[i] * query[i];
+        q[i] = query[i];
+    }
+    mean = ex / m;
+    sdv = ex2 / m;
+    sdv = sqrt(sdv - mean * mean);
+    for (int i = 0; i < m; i++) q[i] = (q[i] - mean) / sdv;
+    ucr_lemire(q, m, r, l, u);
+    for (int i = 0; i < m; i++) {
+        qt[i].v = q[i];
+        qt[i].i = i;
+    }
+    qsort(qt, (size_t)m, sizeof(ucr_pair), ucr_cmp_abs_desc);
+    for (int i = 0; i < m; i++) {
+        int o = qt[i].i;
+        order[i] = o;
+        qo[i] = q[o];
+        uo[i] = u[o];
+        lo[i] = l[o];
+    }
+
+    int64_t pos = 0, loc = 0;
+    int64_t kim = 0, keogh = 0, keogh2 = 0, it = 0, scanned = 0;
+    int done = 0, ep = 0;
+    while (!done) {
+        if (it == 0) {
+            for (int k = 0; k < m - 1; k++)
+                if (pos < n) buffer[k] = data[pos++];
+        } else {
+            for (int k = 0; k < m - 1; k++) buffer[k] = buffer[EPOCH - m + 1 + k];
+        }
+        ep = m - 1;
+        while (ep < EPOCH) {
+            if (pos >= n) break;
+            buffer[ep] = data[pos++];
+            ep++;
+        }
+        if (ep <= m - 1) {
+            done = 1;
+        } else {
+            ucr_lemire(buffer, ep, r, l_buff, u_buff);
+            ex = 0;
+            ex2 = 0;
+            for (int i = 0; i < ep; i++) {
+                double d = buffer[i];
+                ex += d;
+                ex2 += d * d;
+                t[i % m] = d;
+                t[(i % m) + m] = d;
+                if (i >= m - 1) {
+                    mean = ex / m;
+                    sdv = ex2 / m;
+                    sdv = sqrt(sdv - mean * mean);
+                    int j = (i + 1) % m;
+                    int I = i - (m - 1);
+                    double lb_kim = ucr_lb_kim_hierarchy(t, q, j, m, mean, sdv, bsf);
+                    if (lb_kim < bsf) {
+                        double lb_k = ucr_lb_keogh_cumulative(order, t, uo, lo, cb1, j, m,
+                                                              mean, sdv, bsf);
+                        if (lb_k < bsf) {
+                            for (int k = 0; k < m; k++) tz[k] = (t[(k + j)] - mean) / sdv;
+                            double lb_k2 = ucr_lb_keogh_data_cumulative(
+                                order, qo, cb2, l_buff + I, u_buff + I, m, mean, sdv, bsf);
+                            if (lb_k2 < bsf) {
+                                const double* src = lb_k > lb_k2 ? cb1 : cb2;
+                                cb[m - 1] = src[m - 1];
+                                for (int k = m - 2; k >= 0; k--) cb[k] = cb[k + 1] + src[k];
+                                double dd = ucr_dtw(tz, q, cb, m, r, bsf, cost, cost_prev);
+                                if (dd < bsf) {
+                                    bsf = dd;
+                                    loc = it * (EPOCH - m + 1) + i - m + 1;
+                                }
+                            } else
+                                keogh2++;
+                        } else
+                            keogh++;
+                    } else
+                        kim++;
+                    ex -= t[j];
+                    ex2 -= t[j] * t[j];
+                }
+            }
+            if (ep < EPOCH)
+                done = 1;
+            else
+                it++;
+        }
+    }
+    scanned = it * (EPOCH - m + 1) + ep;
+    if (dist) *dist = bsf;
+    if (counts) {
+        counts[0] = kim;
+        counts[1] = keogh;
+        counts[2] = keogh2;
+        counts[3] = scanned;
+    }
+    free(q); free(qo); free(uo); free(lo); free(u); free(l);
+    free(cb); free(cb1); free(cb2); free(t); free(tz); free(buffer);
+    free(u_buff); free(l_buff); free(cost); free(cost_prev); free(order); free(qt);
+    return loc;
+}
+
+/* UCR_ED_BIN.cpp:55-65 distance, :159-213 search loop. */
+int64_t or_ucr_ed(const double* data, int64_t n, const double* query,
+                  int32_t m, double* dist) {
+    double *Q = malloc(sizeof(double) * m), *T = malloc(sizeof(double) * 2 * m);
+    int* order = malloc(sizeof(int) * m);
+    ucr_pair* qt = malloc(sizeof(ucr_pair) * m);
+    double ex = 0, ex2 = 0, mean, sdv, bsf = UCR_INF;
+    for (int i = 0; i < m; ++i) {
+        ex += query[i];
+        ex2 += query[i] * query[i];
+    }
+    mean = ex / m;
+    sdv = ex2 / m;
+    sdv = sqrt(sdv - mean * mean);
+    for (int i = 0; i < m; i++) {
+        qt[i].v = (query[i] - mean) / sdv;
+        qt[i].i = i;
+    }
+    qsort(qt, (size_t)m, sizeof(ucr_pair), ucr_cmp_abs_desc);
+    for (int i = 0; i < m; i++) {
+        Q[i] = qt[i].v;
+        order[i] = qt[i].i;
+    }
+    int64_t loc = 0;
+    ex = ex2 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double d = data[i];
+        ex += d;
+        ex2 += d * d;
+        T[i % m] = d;
+        T[(i % m) + m] = d;
+        if (i >= m - 1) {
+            int j = (int)((i + 1) % m);
+            mean = ex / m;
+            sdv = ex2 / m;
+            sdv = sqrt(sdv - mean * mean);
+            double sum = 0;
+            for (int k = 0; k < m && sum < bsf; k++) {
+                double x = (T[(order[k] + j)] - mean) / sdv;
+                sum += (x - Q[k]) * (x - Q[k]);
+            }
+            if (sum < bsf) {
+                bsf = sum;
+                loc = i - m + 1;
+            }
+            ex -= T[j];
+            ex2 -= T[j] * T[j];
+        }
+    }
+    if (dist) *dist = bsf;
+    free(Q); free(T); free(order); free(qt);
+    return loc;
+}

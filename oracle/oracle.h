@@ -1,0 +1,125 @@
+/*
+ * oracle.h -- CPU restatement of the reference's z-normalised subsequence
+ * search path (gravitino/cudadtw, include/ *.cuh), plus a port of the UCR
+ * Suite cascade the reference benchmarks against.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path (cudadtw_amd, libmdtw.so) never links, loads or calls it.
+ *
+ * Every function names the reference file:line it restates.  Floating-point
+ * policy: compiled with -ffp-contract=off; every fused multiply-add is an
+ * explicit fmaf()/fma(), so the rounding sequence is fixed and identical to
+ * the HIP kernels that claim bit-exact parity with it.
+ *
+ * Parity-pinning status of each function (what checks it, in tests/):
+ *   or_znormalize_f32/f64   closed form: mean 0, population sigma 1 (FP64).
+ *   or_envelope             brute-force clamped max/min, exact equality.
+ *   or_stats_fp64           numpy two-pass per-window mean/std (independent).
+ *   or_stats_refcompat      exact small-integer data (FP32 finalize exact),
+ *                           epoch-boundary invariance, constant series -> 1.
+ *   or_lb_kim               generic FP64 corner-block DTW on tiny inputs;
+ *                           LB_Kim <= cDTW on every window.
+ *   or_lb_keogh             numpy closed form; LB_Keogh <= cDTW.
+ *   or_cdtw                 FP64 full-matrix brute force; W=0 -> ED^2;
+ *                           W >= M-1 -> textbook unconstrained DTW.
+ *   or_ldtw                 path property check (every move is the greedy
+ *                           choice under the reference's tie rules, the value
+ *                           is the path's cost) and LDTW >= cDTW.
+ *   or_ed_f64               numpy direct sum; UCR_ED_BIN location (golden).
+ *   or_topk_greedy          brute-force enumeration on tiny inputs.
+ *   or_ucr_dtw / or_ucr_ed  UCR_DTW_BIN / UCR_ED_BIN compiled from the
+ *                           reference sources (oracle/_ref), location golden.
+ */
+#ifndef MDTW_ORACLE_H
+#define MDTW_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* metric: 1 = squared Euclidean local cost, 0 = Manhattan (|d|). */
+
+/* stats.cuh:8-28, value_t = float (GPU-dtw.cu:238). */
+void or_znormalize_f32(float* x, int32_t n);
+/* stats.cuh:8-28, value_t = double (GPU-ed.cu:62). */
+void or_znormalize_f64(double* x, int32_t n);
+
+/* bounds.cuh:13-64 (Lemire two-deque envelope). */
+void or_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m);
+
+/* stats.cuh:85-142: FP64 prefix per 10^7-sample epoch with M-1 overlap,
+ * FP64 window difference / M cast to FP32, FP32 finalize fmaf(-X, X, X2),
+ * sigma := 1 when the FP32 variance is not positive.  epoch <= 0 means the
+ * reference's 10,000,000. */
+void or_stats_refcompat(const float* s, int64_t n, int32_t m, int64_t epoch,
+                        float* avg, float* sd);
+
+/* Accurate window statistics: two-pass FP64 mean and population sigma per
+ * window; sigma := 1 where the variance is exactly 0 (stats.cuh:50 guard).
+ * avg64/sd64 may be NULL. */
+void or_stats_fp64(const float* s, int64_t n, int32_t m, float* avg, float* sd,
+                   double* avg64, double* sd64);
+
+/* bounds.cuh:152-281 (idx != NULL) and :283-407 (idx == NULL: all windows). */
+void or_lb_kim(const float* s, const float* avg, const float* sd,
+               const float* zq, int32_t m, const int32_t* idx, int64_t count,
+               int32_t metric, float* out);
+
+/* bounds.cuh:111-144 (gathered) / :70-109 (all windows). */
+void or_lb_keogh(const float* s, const float* avg, const float* sd,
+                 const float* lo, const float* up, int32_t m,
+                 const int32_t* idx, int64_t count, int32_t metric, float* out);
+
+/* cdtw.cuh:23-88 sequential_cdtw.  fused = 1 rounds each cell once
+ * (fmaf(d, d, min), the reference GPU kernels' arithmetic, cdtw.cuh:339-350);
+ * fused = 0 rounds the square then the add (the reference's host build).
+ * Output is the squared distance D[M][M]. */
+void or_cdtw(const float* zq, const float* s, const float* avg, const float* sd,
+             int32_t m, const int32_t* idx, int64_t count, int32_t w,
+             int32_t metric, int32_t fused, float* out);
+
+/* FP64 companion of or_cdtw (z-normalisation and DP in double). */
+void or_cdtw_f64(const double* zq, const float* s, const double* avg,
+                 const double* sd, int32_t m, const int32_t* idx, int64_t count,
+                 int32_t w, int32_t metric, double* out);
+
+/* ldtw.cuh:74-127 z_serial_ldtw, for windows idx[0..count) (idx NULL: 0..).
+ * moves (optional, count*(2m-2) bytes): 0 = diagonal, 1 = down, 2 = right,
+ * 3 = no step (path already at the end). */
+void or_ldtw(const float* zq, const float* s, const float* avg, const float* sd,
+             int32_t m, const int32_t* idx, int64_t count, int32_t w,
+             int32_t metric, float* out, uint8_t* moves);
+
+/* z-normalised Euclidean distance squared, all in FP64 (the quantity
+ * ed.cuh:61-69,149 computes through cuFFT: sqrt(2M + bsf) = ED). */
+void or_ed_f64(const double* zq, const float* s, int64_t n, int32_t m,
+               const double* avg64, const double* sd64, double* out);
+
+/* ldtw.cuh:177-192 determine_min: strict '<', lowest index wins ties. */
+int64_t or_argmin(const float* x, int64_t n);
+
+/* Greedy top-k with exclusion zone: repeatedly take the smallest (distance,
+ * index) not within |i - j| < excl of an earlier pick.  Returns the number of
+ * picks (< k only when every window is excluded). */
+int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
+                       int64_t* out_idx, float* out_dist);
+
+/* Port of UCR_DTW_BIN.cpp:129-354,527-707 (LB_Kim hierarchy, LB_Keogh EQ and
+ * EC, early-abandoning banded DTW, EPOCH streaming).  Returns the location;
+ * *dist gets the squared distance; counts (optional) = {kim, keogh, keogh2,
+ * dtw} pruning counters as UCR prints them. */
+int64_t or_ucr_dtw(const double* data, int64_t n, const double* query,
+                   int32_t m, int32_t r, double* dist, int64_t* counts);
+
+/* Port of UCR_ED_BIN.cpp:55-65,188-213 (early-abandoning z-normalised ED). */
+int64_t or_ucr_ed(const double* data, int64_t n, const double* query,
+                  int32_t m, double* dist);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

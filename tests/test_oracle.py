@@ -1,0 +1,384 @@
+"""Pins the CPU oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something that is not a retyping of
+its own formula: brute force, closed forms, special cases that reduce to
+textbook routines, invariants, and the golden locations printed by the
+reference's UCR_DTW_BIN / UCR_ED_BIN built from source (tests/golden).
+The reference's tests are invariant printers (tests/test_dtw.cu:156-158,
+tests/test_dtwbounds.cu:129-149); their invariants are asserted here.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from cudadtw_amd import synth
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "ucr_golden.json")
+
+
+# ---- independent FP64 references -------------------------------------------
+def np_window_stats(s, m):
+    w = np.lib.stride_tricks.sliding_window_view(s.astype(np.float64), m)
+    return w.mean(axis=1), w.std(axis=1)
+
+
+def full_dtw(a, b, w, l2=True):
+    """Textbook O(M^2) DTW on the full matrix with |i-j| <= w (float64)."""
+    m = a.size
+    D = np.full((m + 1, m + 1), np.inf)
+    D[0, 0] = 0.0
+    for i in range(1, m + 1):
+        for j in range(max(1, i - w), min(m, i + w) + 1):
+            c = (a[i - 1] - b[j - 1]) ** 2 if l2 else abs(a[i - 1] - b[j - 1])
+            D[i, j] = c + min(D[i - 1, j - 1], D[i - 1, j], D[i, j - 1])
+    return D[m, m]
+
+
+def corner_lb_kim(q, z, l2=True):
+    """LB_Kim restated generically: min over the exit boundary of the front
+    5x5 unconstrained DTW plus the back 5x5 DTW entered anywhere on its
+    first row / column (bounds.cuh:283-407), via a generic DP."""
+    def cost(a, b):
+        return (a - b) ** 2 if l2 else abs(a - b)
+
+    F = np.full((5, 5), np.inf)
+    for i in range(5):
+        for j in range(5):
+            c = cost(q[i], z[j])
+            if i == 0 and j == 0:
+                F[i, j] = c
+            else:
+                best = min(F[i - 1, j - 1] if i and j else np.inf, F[i - 1, j] if i else np.inf,
+                           F[i, j - 1] if j else np.inf)
+                F[i, j] = c + best
+    front = min(F[:, 4].min(), F[4, :].min())
+    qb, zb = q[-5:], z[-5:]
+    Bk = np.full((5, 5), np.inf)
+    for i in range(5):
+        for j in range(5):
+            c = cost(qb[i], zb[j])
+            if i == 0 or j == 0:
+                Bk[i, j] = c
+            else:
+                Bk[i, j] = c + min(Bk[i - 1, j - 1], Bk[i - 1, j], Bk[i, j - 1])
+    return front + Bk[4, 4]
+
+
+def brute_envelope(q, w):
+    m = q.size
+    lo = np.array([q[max(0, i - w):min(m, i + w + 1)].min() for i in range(m)], np.float32)
+    up = np.array([q[max(0, i - w):min(m, i + w + 1)].max() for i in range(m)], np.float32)
+    return lo, up
+
+
+def setup_windows(n=3000, m=64, seed=0, kind="walk"):
+    if kind == "walk":
+        s = synth.random_walk(n, seed)
+    else:
+        s = synth.ecg_like(n, seed)
+    q = O.znormalize_f32(synth.random_walk(m, seed + 1))
+    avg, sd, a64, s64 = O.stats_fp64(s, m, want64=True)
+    return s, q, avg, sd, a64, s64
+
+
+# ---- znormalize / envelope ---------------------------------------------------
+def test_znormalize_closed_form():
+    x = synth.random_walk(1000, 3)
+    z = O.znormalize_f32(x).astype(np.float64)
+    assert abs(z.mean()) < 1e-5
+    assert abs(z.std() - 1.0) < 1e-5
+    z64 = O.znormalize_f64(x.astype(np.float64))
+    assert abs(z64.mean()) < 1e-12 and abs(z64.std() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("trial", range(60))
+def test_envelope_equals_bruteforce(trial):
+    rng = np.random.default_rng(trial)
+    m = int(rng.integers(1, 200))
+    w = int(rng.integers(0, m + 3))
+    if trial % 3 == 0:
+        q = rng.integers(-3, 4, size=m).astype(np.float32)  # repeated values
+    else:
+        q = rng.standard_normal(m).astype(np.float32)
+    lo, up = O.envelope(q, w)
+    blo, bup = brute_envelope(q, min(w, m - 1))
+    assert np.array_equal(lo, blo) and np.array_equal(up, bup)
+
+
+# ---- window statistics -------------------------------------------------------
+def test_stats_fp64_matches_numpy():
+    s = synth.random_walk(20000, 7)
+    for m in (11, 128, 1000):
+        avg, sd, a64, s64 = O.stats_fp64(s, m, want64=True)
+        mu, sigma = np_window_stats(s, m)
+        assert np.allclose(a64, mu, rtol=0, atol=1e-9 * np.abs(mu).max())
+        assert np.max(np.abs(s64 - sigma) / sigma) < 1e-9
+        assert np.array_equal(avg, mu.astype(np.float32))
+
+
+def test_stats_fp64_constant_window_sigma_one():
+    s = np.concatenate([np.full(300, 5.5, np.float32), synth.random_walk(300, 1)])
+    avg, sd = O.stats_fp64(s, 100)
+    assert np.all(sd[:201] == 1.0) and np.all(avg[:201] == 5.5)
+
+
+def test_stats_refcompat_exact_on_small_integers():
+    # Small integers: every prefix, window mean and FP32 finalize is exact, so
+    # the reference arithmetic must give the exact population statistics.
+    rng = np.random.default_rng(0)
+    s = rng.integers(-8, 9, size=5000).astype(np.float32)
+    m = 16
+    avg, sd = O.stats_refcompat(s, m)
+    mu, sigma = np_window_stats(s, m)
+    assert np.array_equal(avg, mu.astype(np.float32))
+    var = (sigma ** 2).astype(np.float32)
+    expect = np.where(var > 0, np.sqrt(var), 1.0).astype(np.float32)
+    assert np.max(np.abs(sd - expect) / expect) < 2e-7
+
+
+def test_stats_refcompat_epochs_cover_every_window():
+    # Tiny epochs exercise the epoch/overlap/merge plan of stats.cuh:120-141;
+    # on exact data the result cannot depend on where epochs split.
+    rng = np.random.default_rng(1)
+    s = rng.integers(-4, 5, size=3001).astype(np.float32)
+    m = 32
+    ref = O.stats_refcompat(s, m)
+    for epoch in (34, 35, 100, 257, 3001, 10 ** 7):
+        got = O.stats_refcompat(s, m, epoch)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), epoch
+
+
+def test_stats_refcompat_loses_precision_on_offset_data():
+    # The known defect (FP32 finalize of E[x^2] - E[x]^2): on offset data its
+    # sigma is far off, the FP64 mode is not.
+    s = synth.metal_like(1 << 16, 3, offset=7000.0)
+    m = 512
+    _, sd_ref = O.stats_refcompat(s, m)
+    _, sd64 = O.stats_fp64(s, m)
+    _, sigma = np_window_stats(s, m)
+    assert np.median(np.abs(sd64 - sigma) / sigma) < 1e-6
+    assert np.median(np.abs(sd_ref - sigma) / sigma) > 1e-3
+
+
+# ---- cDTW ----------------------------------------------------------------------
+@pytest.mark.parametrize("w", [0, 1, 3, 7, 20])
+def test_cdtw_matches_full_matrix_fp64(w):
+    s, q, avg, sd, a64, s64 = setup_windows(n=400, m=24)
+    idx = np.arange(0, 377, 13)
+    got = O.cdtw(q, s, avg, sd, w, idx=idx)
+    for g, i in zip(got, idx):
+        z = (s[i:i + 24].astype(np.float64) - avg[i]) / sd[i]
+        ref = full_dtw(q.astype(np.float64), z, w)
+        assert abs(g - ref) <= 1e-5 * max(ref, 1e-3)
+
+
+def test_cdtw_w0_is_squared_ed():
+    s, q, avg, sd, a64, s64 = setup_windows(n=2000, m=64)
+    got = O.cdtw(q, s, avg, sd, 0)
+    L = s.size - 64 + 1
+    z = (np.lib.stride_tricks.sliding_window_view(s.astype(np.float64), 64)
+         - avg[:, None].astype(np.float64)) / sd[:, None].astype(np.float64)
+    ed2 = ((q.astype(np.float64)[None, :] - z) ** 2).sum(axis=1)
+    assert got.shape == (L,)
+    assert np.max(np.abs(got - ed2) / np.maximum(ed2, 1e-3)) < 1e-5
+
+
+def test_cdtw_full_band_is_unconstrained_dtw():
+    s, q, avg, sd, a64, s64 = setup_windows(n=300, m=20)
+    for i in (0, 50, 133, 280):
+        z = (s[i:i + 20].astype(np.float64) - avg[i]) / sd[i]
+        ref = full_dtw(q.astype(np.float64), z, 10 ** 6)
+        for w in (19, 25, 1000):
+            g = O.cdtw(q, s, avg, sd, w, idx=[i])[0]
+            assert abs(g - ref) <= 1e-5 * ref
+
+
+def test_cdtw_fused_and_host_rounding_agree():
+    # cdtw.cuh's GPU kernels (fused square-add) vs its host build (separate
+    # roundings): the reference's own test_dtw tolerates 1e-4 absolute.
+    s, q, avg, sd, *_ = setup_windows(n=5000, m=128)
+    a = O.cdtw(q, s, avg, sd, 13, fused=1)
+    b = O.cdtw(q, s, avg, sd, 13, fused=0)
+    assert np.max(np.abs(a - b) / b) < 1e-5
+
+
+def test_cdtw_l1_matches_full_matrix():
+    s, q, avg, sd, *_ = setup_windows(n=300, m=16)
+    for i in (0, 77, 284):
+        z = (s[i:i + 16].astype(np.float64) - avg[i]) / sd[i]
+        ref = full_dtw(q.astype(np.float64), z, 4, l2=False)
+        g = O.cdtw(q, s, avg, sd, 4, idx=[i], metric=0)[0]
+        assert abs(g - ref) <= 1e-5 * ref
+
+
+def test_cdtw_fp64_companion():
+    s, q, avg, sd, a64, s64 = setup_windows(n=2000, m=64)
+    f32 = O.cdtw(q, s, avg, sd, 6)
+    f64 = O.cdtw_f64(q.astype(np.float64), s, a64, s64, 6)
+    assert np.max(np.abs(f32 - f64) / f64) < 1e-4
+
+
+# ---- lower bounds ----------------------------------------------------------------
+@pytest.mark.parametrize("l2", [1, 0])
+def test_lb_kim_matches_generic_corner_dp(l2):
+    s, q, avg, sd, *_ = setup_windows(n=600, m=40)
+    idx = np.arange(0, 561, 7)
+    got = O.lb_kim(s, avg, sd, q, idx=idx, metric=l2)
+    for g, i in zip(got, idx):
+        z = (s[i:i + 40].astype(np.float64) - avg[i]) / sd[i]
+        ref = corner_lb_kim(q.astype(np.float64), z, bool(l2))
+        assert abs(g - ref) <= 1e-5 * max(ref, 1e-3)
+
+
+def test_lb_keogh_matches_closed_form():
+    s, q, avg, sd, *_ = setup_windows(n=1500, m=50)
+    lo, up = O.envelope(q, 5)
+    got = O.lb_keogh(s, avg, sd, lo, up)
+    z = (np.lib.stride_tricks.sliding_window_view(s.astype(np.float64), 50)
+         - avg[:, None]) / sd[:, None]
+    e = np.where(z > up, z - up, np.where(z < lo, z - lo, 0.0))
+    ref = (e ** 2).sum(axis=1)
+    assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-3)) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["walk", "ecg"])
+def test_lower_bounds_below_cdtw(kind):
+    # tests/test_dtwbounds.cu:129-132 and :146-149
+    s, q, avg, sd, *_ = setup_windows(n=4000, m=128, kind=kind)
+    w = 13
+    lo, up = O.envelope(q, w)
+    d = O.cdtw(q, s, avg, sd, w)
+    kim = O.lb_kim(s, avg, sd, q)
+    keo = O.lb_keogh(s, avg, sd, lo, up)
+    assert np.all(kim <= d + 1e-4)
+    assert np.all(keo <= d + 1e-4)
+
+
+# ---- LDTW ---------------------------------------------------------------------------
+def test_ldtw_path_is_greedy_and_valid():
+    s, q, avg, sd, *_ = setup_windows(n=500, m=32)
+    w = 4
+    idx = np.arange(0, 469, 11)
+    vals, moves = O.ldtw(q, s, avg, sd, w, idx=idx, want_moves=True)
+    inf = np.float32(1e10)
+    for v, mv, st in zip(vals, moves, idx):
+        z = ((s[st:st + 32] - avg[st]) / sd[st]).astype(np.float32)
+        i = j = 0
+        cost = np.float32((q[0] - z[0]) ** 2)
+        for step in mv:
+            if step == 3:
+                break
+            diag = np.float32((q[i + 1] - z[j + 1]) ** 2) if i + 1 < 32 and j + 1 < 32 else inf
+            down = np.float32((q[i + 1] - z[j]) ** 2) if i + 1 < 32 and abs(i - j + 1) <= w else inf
+            rght = np.float32((q[i] - z[j + 1]) ** 2) if j + 1 < 32 and abs(j - i + 1) <= w else inf
+            # the chosen move is the reference's tie-ruled minimum (ldtw.cuh:109-123)
+            if step == 0:
+                assert diag < down and diag < rght
+                cost += diag; i += 1; j += 1
+            elif step == 1:
+                assert not diag < down and down < rght
+                cost += down; i += 1
+            else:
+                assert not (diag < down and diag < rght) and not (not diag < down and down < rght)
+                cost += rght; j += 1
+            assert abs(i - j) <= w
+        assert i == 31 and j == 31
+        assert abs(cost - v) <= 1e-5 * max(v, 1e-3)
+
+
+def test_ldtw_not_below_cdtw():
+    s, q, avg, sd, *_ = setup_windows(n=3000, m=64)
+    for w in (2, 6, 12):
+        l = O.ldtw(q, s, avg, sd, w)
+        d = O.cdtw(q, s, avg, sd, w)
+        assert np.all(l >= d - 1e-4 * d)
+
+
+# ---- ED, argmin, top-k -------------------------------------------------------------
+def test_ed_matches_numpy():
+    s, q, avg, sd, a64, s64 = setup_windows(n=3000, m=100)
+    q64 = O.znormalize_f64(synth.random_walk(100, 1).astype(np.float64))
+    got = O.ed_f64(q64, s, a64, s64)
+    z = (np.lib.stride_tricks.sliding_window_view(s.astype(np.float64), 100)
+         - a64[:, None]) / s64[:, None]
+    ref = ((q64[None, :] - z) ** 2).sum(axis=1)
+    assert np.allclose(got, ref, rtol=1e-12)
+
+
+def test_argmin_lowest_index_on_ties():
+    x = np.array([3, 1, 2, 1, 1], np.float32)
+    assert O.argmin(x) == 1
+
+
+def test_topk_greedy_bruteforce():
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        n = int(rng.integers(5, 40))
+        d = rng.integers(0, 6, size=n).astype(np.float32)  # many ties
+        k = int(rng.integers(1, 5))
+        excl = int(rng.integers(0, 5))
+        idx, dist = O.topk_greedy(d, k, excl)
+        # brute force of the greedy definition
+        order = sorted(range(n), key=lambda i: (d[i], i))
+        picks = []
+        for i in order:
+            if len(picks) == k:
+                break
+            if all(abs(i - p) >= excl for p in picks):
+                picks.append(i)
+        assert list(idx) == picks
+
+
+# ---- golden: the reference's UCR comparators built from source ---------------------
+def _golden_cases():
+    with open(GOLDEN) as f:
+        g = json.load(f)
+    s, q, w = synth.config1()
+    yield "config1_random_walk", s, q, w, g
+    e = synth.ecg_like(1 << 18, 5)
+    yield "ecg_2p18_m128", e, synth.ecg_query(128, 2), 13, g
+    q2, _ = synth.cut_query(e, 256, 9)
+    yield "ecg_2p18_m256_cut", e, q2, 26, g
+    mt = synth.metal_like(1 << 18, 3)
+    qm, _ = synth.cut_query(mt, 512, 4)
+    yield "metal_2p18_m512_cut", mt, qm, 51, g
+
+
+@pytest.mark.parametrize("case", [c[0] for c in _golden_cases()])
+def test_golden_ucr_locations(case):
+    for name, s, q, w, g in _golden_cases():
+        if name != case:
+            continue
+        ref = g[name]
+        sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+        assert sha(s) == ref["stream_sha256"] and sha(q) == ref["query_sha256"]
+        # UCR cascade port == UCR_DTW_BIN
+        loc, d2, _ = O.ucr_dtw(s.astype(np.float64), q.astype(np.float64), w)
+        assert loc == ref["ucr_dtw"]["location"]
+        assert abs(np.sqrt(d2) - ref["ucr_dtw"]["distance"]) <= 1e-5 * ref["ucr_dtw"]["distance"]
+        # the restated reference path (FP64 stats, FP32 banded DP, argmin).
+        # UCR normalises the query in FP64; the reference's GPU driver does it
+        # in FP32 (stats.cuh:8-28 with value_t = float), which moves offset
+        # queries' distances by ~1e-4: the location must agree either way, the
+        # distance tightly when the query is normalised as UCR does.
+        avg, sd = O.stats_fp64(s, q.size)
+        zq = O.znormalize_f32(q)
+        prof = O.cdtw(zq, s, avg, sd, w)
+        assert O.argmin(prof) == ref["ucr_dtw"]["location"]
+        assert abs(np.sqrt(prof.min()) / ref["ucr_dtw"]["distance"] - 1) < 1e-3
+        zq64 = O.znormalize_f64(q.astype(np.float64)).astype(np.float32)
+        prof = O.cdtw(zq64, s, avg, sd, w)
+        i = O.argmin(prof)
+        assert i == ref["ucr_dtw"]["location"]
+        assert abs(np.sqrt(prof[i]) - ref["ucr_dtw"]["distance"]) <= 2e-5 * ref["ucr_dtw"]["distance"]
+        # ED
+        eloc, ed2 = O.ucr_ed(s.astype(np.float64), q.astype(np.float64))
+        assert eloc == ref["ucr_ed"]["location"]
+        q64 = O.znormalize_f64(q.astype(np.float64))
+        _, _, a64, s64 = O.stats_fp64(s, q.size, want64=True)
+        ed = O.ed_f64(q64, s, a64, s64)
+        assert int(np.argmin(ed)) == ref["ucr_ed"]["location"]
