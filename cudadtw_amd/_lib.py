@@ -70,6 +70,7 @@ SIGNATURES = {
     "mdtw_device_info": (C.c_int, [_P, C.c_char_p, _I32, C.POINTER(_I32)]),
     "mdtw_set_query": (C.c_int, [_P, _FP, _FP, _FP, _I32, _I32]),
     "mdtw_znormalize": (C.c_int, [_FP, _I32]),
+    "mdtw_znormalize_f64": (C.c_int, [C.POINTER(C.c_double), _I32]),
     "mdtw_envelope": (C.c_int, [_FP, _I32, _FP, _FP, _I32]),
     "mdtw_avg_std": (C.c_int, [_P, _P, _I64, _I32, _P, _P, _I32]),
     "mdtw_lb_kim": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, _I32]),

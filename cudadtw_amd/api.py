@@ -34,9 +34,16 @@ def _fp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
-def znormalize(series) -> np.ndarray:
-    """stats.cuh:8-28 znormalize<float>: returns the z-normalised FP32 copy,
-    bit-identical to the reference's host routine (FP32 sums in order)."""
+def znormalize(series, double: bool = False) -> np.ndarray:
+    """stats.cuh:8-28 znormalize: returns the z-normalised FP32 copy,
+    bit-identical to the reference's host routine.  double=False is
+    value_t = float (the cDTW / LDTW drivers, GPU-dtw.cu:67); double=True is
+    value_t = double (the ED driver, GPU-ed.cu:62), rounded to FP32 after."""
+    if double:
+        x = np.ascontiguousarray(series, dtype=np.float64).copy()
+        check(_lib.load().mdtw_znormalize_f64(x.ctypes.data_as(C.POINTER(C.c_double)), x.size),
+              "znormalize")
+        return x.astype(np.float32)
     x = np.ascontiguousarray(series, dtype=np.float32).copy()
     check(_lib.load().mdtw_znormalize(_fp(x), x.size), "znormalize")
     return x

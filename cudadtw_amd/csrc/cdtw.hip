@@ -226,14 +226,12 @@ int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain, 
     const bool l2 = metric != 0;
     const int64_t count_hint = a.idx_count_from_ctrl ? -1 : a.count;
     if (!a.idx_count_from_ctrl && a.count <= 0) return MDTW_OK;
-    BandLaunchFn fn = force_wide ? nullptr : band_lookup(w, l2);
+    BandLaunchFn fn = force_wide || a.qx == nullptr ? nullptr : band_lookup(w, l2);
     if (fn) {
-        unsigned grid;
-        if (a.persistent)
-            grid = static_cast<unsigned>(cus * 16);
-        else
-            grid = static_cast<unsigned>((count_hint + kWave - 1) / kWave);
-        return fn(st, grid, a);
+        const int64_t per_block = static_cast<int64_t>(kBandWaves) * kWave;
+        int64_t grid = static_cast<int64_t>(cus) * 8;
+        if (!a.persistent) grid = std::min(grid, (count_hint + per_block - 1) / per_block);
+        return fn(st, static_cast<unsigned>(grid), a);
     }
     if (w > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
     BandArgs b = a;

@@ -2,18 +2,19 @@
 // registers of one lane (included by the cdtw_band_*.hip instantiation units).
 //
 // Mapping.  Lane = window (64 consecutive windows per wave, so stream reads
-// coalesce).  The DP runs column by column over the window's samples j; the
-// band of column j is held as E[t] = D[j - W + t][j], t = 0..2W.  In these
-// coordinates the three predecessors of E[t] are E[t] and E[t+1] of the
-// previous column and E[t-1] of this column, so the update is in place in
-// increasing t with no data movement:
+// coalesce; 4 waves per workgroup).  The DP runs column by column over the
+// window's samples j; the band of column j is held as E[t] = D[j - W + t][j],
+// t = 0..2W.  In these coordinates the three predecessors of E[t] are E[t]
+// and E[t+1] of the previous column and E[t-1] of this column, so the update
+// is in place in increasing t with no data movement:
 //     E[t] = fmaf(d, d, min(E[t-1], E[t], E[t+1])),   d = q[j-W+t] - z_j
 // (v_sub, v_min3, v_fmac: three VALU slots per cell).  The query index
-// j - W + t is wave-uniform, so the query is read with scalar loads from a
-// copy padded with W +inf sentinels on each side (out-of-matrix cells cost
-// +inf, which min() then ignores).  z_j = (x_j - mu) / sigma is one IEEE
-// division per column per lane, amortised over 2W + 1 cells.  The answer is
-// E[W] after column M - 1.
+// j - W + t is wave-uniform, so the query -- padded with W +inf sentinels on
+// each side (out-of-matrix cells cost +inf, which min() then ignores) -- is
+// read with scalar loads into SGPRs: no per-lane copies, no LDS bandwidth (a
+// broadcast LDS read still moves 64 lanes' worth of data).  z_j =
+// (x_j - mu) / sigma is computed once per column per lane (see band_z),
+// amortised over 2W + 1 cells.  The answer is E[W] after column M - 1.
 //
 // The arithmetic per cell is that of cdtw.cuh:339-350 (z-normalise by
 // division, q - z, fused square-add onto the minimum), so results are
@@ -30,10 +31,13 @@ namespace mdtw {
 
 using BandLaunchFn = int (*)(hipStream_t, unsigned grid, const BandArgs&);
 
-// Column-block unroll and abandon-check interval (in columns).
+// Column-block unroll, abandon-check interval (columns), query chunk size.
 constexpr int kBandU = 4;
 constexpr int kBandCheck = 16;
-constexpr int kBandChunk = 16;  // query scalars per load group
+constexpr int kBandChunk = 16;
+constexpr int kBandWaves = 4;  // waves per workgroup
+// Narrow-band limit: query window plus prefetch held in scalar registers.
+constexpr int kBandSlideMax = 42;  // spill-free up to W = 17
 
 __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane) {
     unsigned long long b = 0;
@@ -41,18 +45,56 @@ __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane) {
     return static_cast<int64_t>(__shfl(b, 0, kWave));
 }
 
+// z = (x - mu) / sigma, bit-identical to the IEEE quotient: q = a * y with y
+// the correctly rounded 1/sigma, then one Markstein correction
+// fma(fma(-sigma, q, a), y, q).  Outside the exponent range where the
+// residual a - sigma*q is exact (tiny |a|, extreme sigma) the lane divides.
+__device__ __forceinline__ float band_z(float x, float mu, float sg, float y, bool fast) {
+    const float a = x - mu;
+    const float q = a * y;
+    float z = fmaf(fmaf(-sg, q, a), y, q);
+    if (__builtin_expect(!fast || (a != 0.0f && fabsf(a) < 0x1p-100f), 0)) z = a / sg;
+    return z;
+}
+
+// One column of the band update with the query values of this column in
+// q[0..B) (wave-uniform scalars).
+template <int B, bool L2, class Q>
+__device__ __forceinline__ void band_column(float (&E)[B], const Q& q, float zj) {
+    float prev = kInf;
+#pragma unroll
+    for (int t = 0; t < B; ++t) {
+        const float nxt = t + 1 < B ? E[t + 1] : kInf;
+        const float mn = fminf(fminf(prev, E[t]), nxt);
+        const float d = q(t) - zj;
+        const float e = L2 ? fmaf(d, d, mn) : fabsf(d) + mn;
+        E[t] = e;
+        prev = e;
+    }
+}
+
 template <int W, bool L2>
 __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, float sg,
-                                         const float* __restrict__ qx, int32_t m,
-                                         bool& alive, int mode, const Ctrl* ctrl) {
+                                         const float* __restrict__ qx, int32_t m, bool& alive,
+                                         int mode, const Ctrl* ctrl) {
     constexpr int B = 2 * W + 1;
     constexpr int U = kBandU;
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
+    // Narrow bands keep the query window of a whole block of U columns in
+    // scalar registers and slide it by U per block; the next U values are
+    // requested a block ahead, so the only scalar-memory wait is on data
+    // that has had a block of compute to arrive.  (Scalar loads complete out
+    // of order, so every wait is for all of them.)  Wider bands read the
+    // query in chunks of CH scalars per column.
+    constexpr bool kSlide = B + 2 * U - 1 <= kBandSlideMax;
     float E[B];
 #pragma unroll
     for (int t = 0; t < B; ++t) E[t] = kInf;
     E[W] = 0.0f;  // D[-1][-1]
+
+    const float y = 1.0f / sg;
+    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
 
     // Samples of the next U columns; indices are clamped rather than
     // branched so the loads stay unconditional and overlap a whole block.
@@ -60,50 +102,71 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
 #pragma unroll
     for (int u = 0; u < U; ++u) xv[u] = x[u < m ? u : m - 1];
 
+    constexpr int R = kSlide ? B + U - 1 : 1;
+    float qs[R];
+    if constexpr (kSlide) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) qs[k] = qx[k];
+    }
+
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     for (int32_t j0 = 0; j0 < m; j0 += U) {
         float z[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) z[u] = (xv[u] - mu) / sg;
+        for (int u = 0; u < U; ++u) z[u] = band_z(xv[u], mu, sg, y, fast);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int32_t c = j0 + U + u;
             xv[u] = x[c < m ? c : m - 1];
         }
+        if constexpr (kSlide) {
+            // next block's new values: qx[j0 + U + R - U .. + U)
+            float pf[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (j0 + u < m) {
-                const float* __restrict__ qc = qx + (j0 + u);
-                const float zj = z[u];
-                float prev = kInf;
-                // The query band is read CH scalars at a time, one chunk
-                // ahead; the scheduling barrier keeps the scalar loads from
-                // being hoisted together (that would exceed the SGPR file).
-                float qa[CH];
+            for (int u = 0; u < U; ++u) pf[u] = qx[j0 + R + u];
 #pragma unroll
-                for (int i = 0; i < CH; ++i) qa[i] = qc[i];
+            for (int u = 0; u < U; ++u) {
+                if (j0 + u < m) band_column<B, L2>(E, [&](int t) { return qs[u + t]; }, z[u]);
+            }
 #pragma unroll
-                for (int ch = 0; ch < NC; ++ch) {
-                    float qb[CH];
-                    if (ch + 1 < NC) {
+            for (int k = 0; k + U < R; ++k) qs[k] = qs[k + U];
 #pragma unroll
-                        for (int i = 0; i < CH; ++i) qb[i] = qc[(ch + 1) * CH + i];
-                    }
-                    __builtin_amdgcn_sched_barrier(0x96);  // VALU, SALU, VMEM, DS may cross
+            for (int u = 0; u < U; ++u) qs[R - U + u] = pf[u];
+        } else {
 #pragma unroll
-                    for (int i = 0; i < CH; ++i) {
-                        const int t = ch * CH + i;
-                        if (t < B) {
-                            const float nxt = t + 1 < B ? E[t + 1] : kInf;
-                            const float mn = fminf(fminf(prev, E[t]), nxt);
-                            const float d = qa[i] - zj;
-                            const float e = L2 ? fmaf(d, d, mn) : fabsf(d) + mn;
-                            E[t] = e;
-                            prev = e;
+            for (int u = 0; u < U; ++u) {
+                if (j0 + u < m) {
+                    const float* __restrict__ qc = qx + (j0 + u);
+                    const float zj = z[u];
+                    float prev = kInf;
+                    float qa[CH];
+#pragma unroll
+                    for (int i = 0; i < CH; ++i) qa[i] = qc[i];
+#pragma unroll
+                    for (int ch = 0; ch < NC; ++ch) {
+                        float qb[CH];
+                        if (ch + 1 < NC) {
+#pragma unroll
+                            for (int i = 0; i < CH; ++i) qb[i] = qc[(ch + 1) * CH + i];
                         }
-                    }
+                        // keep the scalar loads in chunks (hoisting them all
+                        // would exceed the scalar register file)
+                        __builtin_amdgcn_sched_barrier(0x96);
 #pragma unroll
-                    for (int i = 0; i < CH; ++i) qa[i] = qb[i];
+                        for (int i = 0; i < CH; ++i) {
+                            const int t = ch * CH + i;
+                            if (t < B) {
+                                const float nxt = t + 1 < B ? E[t + 1] : kInf;
+                                const float mn = fminf(fminf(prev, E[t]), nxt);
+                                const float d = qa[i] - zj;
+                                const float e = L2 ? fmaf(d, d, mn) : fabsf(d) + mn;
+                                E[t] = e;
+                                prev = e;
+                            }
+                        }
+#pragma unroll
+                        for (int i = 0; i < CH; ++i) qa[i] = qb[i];
+                    }
                 }
             }
         }
@@ -125,18 +188,22 @@ __device__ __forceinline__ uint32_t band_lane_rank(unsigned long long mask) {
 }
 
 template <int W, bool L2>
-__global__ __launch_bounds__(64) void band_kernel(
+__global__ __launch_bounds__(256) void band_kernel(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
     const float* __restrict__ qx, const int32_t* __restrict__ idx, float* __restrict__ out,
     unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, BandScalars a) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
     int64_t count = a.count;
     if (a.idx_count_from_ctrl)
         count = static_cast<int64_t>(
             __hip_atomic_load(&ctrl->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     unsigned long long n_dp = 0, n_ab = 0;
+    // persistent: waves take 64 windows at a time from an atomic cursor;
+    // otherwise a static grid stride (every wave the same amount of work).
+    const int64_t step = static_cast<int64_t>(gridDim.x) * kBandWaves * kWave;
     int64_t base = a.persistent ? band_next(ctrl, lane)
-                                : static_cast<int64_t>(blockIdx.x) * kWave;
+                                : (static_cast<int64_t>(blockIdx.x) * kBandWaves + wid) * kWave;
     while (base < count) {
         const int64_t g = base + lane;
         const bool valid = g < count;
@@ -165,8 +232,7 @@ __global__ __launch_bounds__(64) void band_kernel(
         }
         n_dp += __popcll(__ballot(valid));
         n_ab += __popcll(__ballot(valid && !alive));
-        if (!a.persistent) break;
-        base = band_next(ctrl, lane);
+        base = a.persistent ? band_next(ctrl, lane) : base + step;
     }
     if (lane == 0 && a.mode != kModeProfile) {
         if (n_dp) atomicAdd(&ctrl->dp, n_dp);
@@ -176,9 +242,10 @@ __global__ __launch_bounds__(64) void band_kernel(
 
 template <int W, bool L2>
 int band_launch(hipStream_t st, unsigned grid, const BandArgs& a) {
-    BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent};
-    hipLaunchKernelGGL((band_kernel<W, L2>), dim3(grid), dim3(kWave), 0, st, a.s, a.avg, a.sd,
-                       a.qx, a.idx, a.out, a.res, a.ctrl, sc);
+    BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent,
+                   0, a.qx_len};
+    hipLaunchKernelGGL((band_kernel<W, L2>), dim3(grid), dim3(kBandWaves * kWave), 0, st, a.s,
+                       a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

@@ -112,7 +112,10 @@ int ensure_qx(mdtw_engine* e, int32_t w) {
     std::vector<float> qx(len, kInf);
     for (int32_t i = 0; i < e->m; ++i) qx[static_cast<size_t>(i) + w] = e->h_q[i];
     int rc = upload(e, e->d_qx, qx.data(), sizeof(float) * len);
-    if (rc == MDTW_OK) e->w_qx = w;
+    if (rc == MDTW_OK) {
+        e->w_qx = w;
+        e->qx_len = static_cast<int32_t>(len);
+    }
     return rc;
 }
 
@@ -282,6 +285,20 @@ int mdtw_znormalize(float* h, int32_t len) {
     return MDTW_OK;
 }
 
+int mdtw_znormalize_f64(double* h, int32_t len) {
+    if (!h || len < 1) return MDTW_ERR_INVALID_ARGUMENT;
+    double avg = 0.0, sd = 0.0;
+    for (int32_t i = 0; i < len; ++i) avg += h[i];
+    avg /= static_cast<double>(len);
+    for (int32_t i = 0; i < len; ++i) sd += h[i] * h[i];
+    sd = std::sqrt(sd / static_cast<double>(len) - avg * avg);
+    for (int32_t i = 0; i < len; ++i) {
+        h[i] -= avg;
+        h[i] /= sd;
+    }
+    return MDTW_OK;
+}
+
 int mdtw_envelope(const float* h, int32_t w, float* lo, float* up, int32_t m) {
     if (!h || !lo || !up || m < 1 || w < 0) return MDTW_ERR_INVALID_ARGUMENT;
     host_envelope(h, w, lo, up, m);
@@ -334,6 +351,7 @@ int mdtw_cdtw(mdtw_engine* e, const float* d_series, const float* d_avg, const f
     a.avg = d_avg;
     a.sd = d_std;
     a.qx = e->d_qx;
+    a.qx_len = e->qx_len;
     a.m = e->m;
     a.idx = d_indices;
     a.count = count;

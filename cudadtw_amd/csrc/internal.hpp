@@ -58,6 +58,7 @@ struct mdtw_engine {
     // Query padded with +inf for the register-band kernel, built per band.
     float* d_qx = nullptr;
     size_t qx_cap = 0;
+    int32_t qx_len = 0;
     int32_t w_qx = -1;
 
     // Growing scratch, reused across calls (no per-call malloc).
@@ -111,6 +112,7 @@ struct BandArgs {
     const float* avg = nullptr;
     const float* sd = nullptr;
     const float* qx = nullptr;  // band kernel: padded query; others: plain query
+    int32_t qx_len = 0;         // length of the padded query buffer
     int32_t m = 0;
     const int32_t* idx = nullptr;
     int32_t idx_count_from_ctrl = 0;
@@ -132,6 +134,8 @@ struct BandScalars {
     int64_t stride;
     int32_t mode;
     int32_t persistent;
+    int32_t lq_len;  // band kernel: LDS length of one shifted query copy
+    int32_t qx_len;  // band kernel: length of the padded query buffer
 };
 
 // ---- kernel launchers (each in its own translation unit) -----------------

@@ -147,6 +147,7 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     a.avg = avg;
     a.sd = sd;
     a.qx = e->d_qx;
+    a.qx_len = e->qx_len;
     a.m = m;
     a.ctrl = e->d_ctrl;
     a.res = res;

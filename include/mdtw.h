@@ -102,8 +102,11 @@ int mdtw_set_query(mdtw_engine* e, const float* h_zquery, const float* h_lower,
                    const float* h_upper, int32_t m, int32_t w);
 
 /* ---- host helpers (bit-exact restatements) ------------------------------ */
-/* stats.cuh:8-28 znormalize<float>: FP32 accumulation in index order. */
+/* stats.cuh:8-28 znormalize<float>: FP32 accumulation in index order
+ * (the cDTW/LDTW drivers, GPU-dtw.cu:67, GPU-ldtw.cu:52). */
 int mdtw_znormalize(float* h_series, int32_t len);
+/* stats.cuh:8-28 znormalize<double> (the ED driver, GPU-ed.cu:62). */
+int mdtw_znormalize_f64(double* h_series, int32_t len);
 /* bounds.cuh:13-64 envelope (Lemire). */
 int mdtw_envelope(const float* h_series, int32_t w, float* h_lower,
                   float* h_upper, int32_t m);

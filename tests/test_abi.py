@@ -25,7 +25,7 @@ def declared_functions():
 def test_library_loads_and_exports_every_declared_symbol():
     lib = _lib.load()
     names = declared_functions()
-    assert len(names) >= 19
+    assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
     # the Python binding covers the same set
@@ -43,6 +43,13 @@ def test_status_strings():
 def test_host_znormalize_bit_exact(seed):
     x = synth.random_walk(int(64 + 100 * seed), seed)
     assert np.array_equal(znormalize(x), O.znormalize_f32(x))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_host_znormalize_f64_bit_exact(seed):
+    x = synth.metal_like(300 + 50 * seed, seed, offset=7000.0)
+    got = znormalize(x, double=True)
+    assert np.array_equal(got, O.znormalize_f64(x.astype(np.float64)).astype(np.float32))
 
 
 @pytest.mark.parametrize("w", [0, 1, 5, 13, 40, 127, 500])
