@@ -79,6 +79,8 @@ SIGNATURES = {
     "mdtw_cdtw_wide": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, _I32, _I32]),
     "mdtw_ldtw": (C.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _I32]),
     "mdtw_ed": (C.c_int, [_P, _P, _P, _P, _P, _I64]),
+    "mdtw_cdtw_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _P,
+                                     _P]),
     "mdtw_search": (C.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64,
                               C.POINTER(_I64), _FP, C.POINTER(SearchInfo)]),
     "mdtw_threshold": (C.c_int, [_P, _P, _P, _I64, _F, C.POINTER(_I64)]),

@@ -235,6 +235,28 @@ class Engine:
                                self._f32(sd, "std"), self._f32(out, "out"), n), "mdtw_ed")
         return out[:n]
 
+    # -- gradient ----------------------------------------------------------
+    def cdtw_backward(self, series, avg, sd, indices, grad, w: int, metric: int = METRIC_L2,
+                      sqrt_dist: bool = True, query_raw=None, want_dist: bool = False):
+        """Query gradient of sum_p grad[p] * dist_p over the windows
+        ``indices`` (mdtw_cdtw_backward; no reference counterpart).  dist_p is
+        the banded cDTW of the engine's query (sqrt when ``sqrt_dist``).  With
+        ``query_raw`` (the raw query whose z-normalisation is the engine's
+        query) the gradient is with respect to the raw query, otherwise with
+        respect to the z-normalised one.  Returns (grad[M], dist or None)."""
+        n = indices.numel()
+        out = self.empty(self.m)
+        dist = self.empty(max(n, 1)) if want_dist else None
+        self._stream()
+        check(self.lib.mdtw_cdtw_backward(
+            self.h, self._f32(series, "series"), self._f32(avg, "avg"), self._f32(sd, "std"),
+            self._i32(indices, "indices"), self._f32(grad, "grad"), n, int(w), int(metric),
+            1 if sqrt_dist else 0,
+            self._f32(query_raw, "query_raw") if query_raw is not None else None,
+            self._f32(dist, "dist") if dist is not None else None, self._f32(out, "grad_query")),
+            "mdtw_cdtw_backward")
+        return out, (dist[:n] if dist is not None else None)
+
     # -- search ------------------------------------------------------------
     def search(self, series, w: int, algo: int = ALGO_CDTW, metric: int = METRIC_L2, k: int = 1,
                excl: int = 0, avg=None, sd=None, stats_mode: int = STATS_FP64) -> SearchResult:

@@ -119,17 +119,6 @@ int ensure_qx(mdtw_engine* e, int32_t w) {
     return rc;
 }
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-        else prev = -1;
-    }
-    ~DeviceGuard() {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
 }  // namespace mdtw
 
 using namespace mdtw;

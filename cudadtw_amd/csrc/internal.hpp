@@ -73,6 +73,25 @@ struct mdtw_engine {
 namespace mdtw {
 
 void set_last_hip_error(hipError_t e);
+
+// MDTW_OK, or MDTW_ERR_HIP with the HIP error kept for mdtw_status_string.
+inline int hip_status(hipError_t err) {
+    if (err == hipSuccess) return MDTW_OK;
+    set_last_hip_error(err);
+    return MDTW_ERR_HIP;
+}
+
+// Makes the engine's device current for one entry point, restores the caller's.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
 int ws_reserve(mdtw_engine* e, size_t bytes, void** out);
 int upload(mdtw_engine* e, void* dst, const void* src, size_t bytes);
 void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m);

@@ -149,6 +149,28 @@ int mdtw_cdtw_wide(mdtw_engine* e, const float* d_series, const float* d_avg,
                    const float* d_std, float* d_out, const int32_t* d_indices,
                    int64_t count, int32_t w, int32_t metric);
 
+/* ---- gradient -------------------------------------------------------------- */
+/* No reference counterpart (the reference computes distances only; this
+ * differentiates its recurrence, cdtw.cuh:23-88).  Gradient with respect to
+ * the query of  sum_p d_grad[p] * dist_p,  where dist_p is the banded cDTW
+ * of the engine's query against window d_indices[p] (squared, or its sqrt
+ * when sqrt_dist, as search distances are reported).  The DP is recomputed
+ * (no state is kept from a forward call) and the optimal warping path walked
+ * back; ties take the diagonal if it is not above either other predecessor,
+ * then (i-1, j) if not above (i, j-1).  With sqrt_dist a zero distance
+ * contributes 0.  d_query_raw NULL: the output d_grad_query (M floats) is the
+ * gradient with respect to the z-normalised query; otherwise d_query_raw must
+ * be the raw query whose z-normalisation was given to mdtw_set_query, and the
+ * gradient is carried through that z-normalisation (population sigma) to the
+ * raw query.  d_dist (may be NULL) receives dist_p, bit-identical to
+ * mdtw_cdtw (sqrt applied when sqrt_dist).  All arrays are device memory. */
+int mdtw_cdtw_backward(mdtw_engine* e, const float* d_series, const float* d_avg,
+                       const float* d_std, const int32_t* d_indices,
+                       const float* d_grad, int64_t count, int32_t w,
+                       int32_t metric, int32_t sqrt_dist,
+                       const float* d_query_raw, float* d_dist,
+                       float* d_grad_query);
+
 /* ---- search -------------------------------------------------------------- */
 /* cdtw.cuh:510-592 prune_cdtw / ed.cuh:71-153 calculate_ed / GPU-ldtw.cu:
  * best (k = 1) or top-k matches of the engine's query in d_series[0..n).

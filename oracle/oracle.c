@@ -377,6 +377,177 @@ void or_cdtw_f64(const double* zq, const float* s, const double* avg,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Query gradient of banded cDTW: the recurrence of cdtw.cuh:23-88 in FP64 on
+ * band storage (row i keeps columns i-w .. i+w), argmin predecessor per cell,
+ * path walked back from the last cell. */
+void or_cdtw_grad_f64(const double* zq, const float* s, const double* avg,
+                      const double* sd, int32_t m, const int32_t* idx,
+                      int64_t count, int32_t w, int32_t metric,
+                      int32_t sqrt_dist, const double* g, const double* q_raw,
+                      double* dist, double* grad) {
+    or_cdtw_grad(zq, s, avg, sd, m, idx, count, w, metric, sqrt_dist, 0, g, q_raw, dist, grad);
+}
+
+/* The optimal path of the FP32 DP of or_cdtw (fused = 1): cells rounded as
+ * fmaf(d, d, min) / fabsf(d) + min with d = q - z, z = (s - avg) / sd in FP32. */
+static void path32_dp(const double* zq, const float* sub, float a, float sdv, int32_t m,
+                      int32_t w, int32_t metric, int bw, float* D, uint8_t* dir, double* z) {
+    for (int j = 0; j < m; ++j) z[j] = (double)((sub[j] - a) / sdv);
+    for (int i = 0; i < m; ++i) {
+        const float qi = (float)zq[i];
+        for (int t = 0; t < bw; ++t) {
+            const int j = i - w + t;
+            float v = INFINITY;
+            uint8_t dr = 3;
+            if (j >= 0 && j < m) {
+                float dg = INFINITY, up = INFINITY, lf = INFINITY;
+                if (i == 0 && j == 0) {
+                    dg = 0.0f;
+                } else {
+                    if (i > 0 && j > 0) dg = D[(size_t)(i - 1) * bw + t];
+                    if (i > 0 && t + 1 < bw) up = D[(size_t)(i - 1) * bw + t + 1];
+                    if (j > 0 && t > 0) lf = D[(size_t)i * bw + t - 1];
+                }
+                float mn;
+                if (dg <= up && dg <= lf) {
+                    mn = dg;
+                    dr = 0;
+                } else if (up <= lf) {
+                    mn = up;
+                    dr = 1;
+                } else {
+                    mn = lf;
+                    dr = 2;
+                }
+                const float d = qi - (float)z[j];
+                v = metric ? fmaf(d, d, mn) : fabsf(d) + mn;
+            }
+            D[(size_t)i * bw + t] = v;
+            dir[(size_t)i * bw + t] = dr;
+        }
+    }
+}
+
+void or_cdtw_grad(const double* zq, const float* s, const double* avg,
+                  const double* sd, int32_t m, const int32_t* idx,
+                  int64_t count, int32_t w, int32_t metric,
+                  int32_t sqrt_dist, int32_t path32, const double* g,
+                  const double* q_raw, double* dist, double* grad) {
+    if (w > m - 1) w = m - 1;
+    const int bw = 2 * w + 1;
+    double* G = (double*)calloc((size_t)m, sizeof(double));
+    double* D = (double*)malloc(sizeof(double) * (size_t)m * bw);
+    float* D32 = (float*)malloc(sizeof(float) * (size_t)m * bw);
+    uint8_t* dir = (uint8_t*)malloc((size_t)m * bw);
+    double* z = (double*)malloc(sizeof(double) * (size_t)m);
+    double* row = (double*)malloc(sizeof(double) * (size_t)m);
+    for (int64_t p = 0; p < count; ++p) {
+        const int64_t ix = idx ? idx[p] : p;
+        if (path32) {
+            path32_dp(zq, s + ix, (float)avg[ix], (float)sd[ix], m, w, metric, bw, D32, dir, z);
+            const float Dm32 = D32[(size_t)(m - 1) * bw + w];
+            double scale = g[p];
+            if (sqrt_dist) scale = Dm32 > 0.0f ? scale * 0.5 / sqrt((double)Dm32) : 0.0;
+            if (dist) dist[p] = sqrt_dist ? (double)sqrtf(Dm32) : (double)Dm32;
+            for (int i = 0; i < m; ++i) row[i] = 0.0;
+            int i = m - 1, j = m - 1;
+            for (;;) {
+                const double d = (double)(float)zq[i] - z[j];
+                row[i] += metric ? 2.0 * d : (d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0));
+                if (i == 0 && j == 0) break;
+                const uint8_t dr = dir[(size_t)i * bw + (j - i + w)];
+                if (dr == 0) {
+                    --i;
+                    --j;
+                } else if (dr == 1) {
+                    --i;
+                } else {
+                    --j;
+                }
+            }
+            for (int r = 0; r < m; ++r) G[r] += scale * row[r];
+            continue;
+        }
+        for (int j = 0; j < m; ++j) z[j] = ((double)s[ix + j] - avg[ix]) / sd[ix];
+        for (int i = 0; i < m; ++i) {
+            for (int t = 0; t < bw; ++t) {
+                const int j = i - w + t;
+                double v = INFINITY;
+                uint8_t dr = 3;
+                if (j >= 0 && j < m) {
+                    double dg = INFINITY, up = INFINITY, lf = INFINITY;
+                    if (i == 0 && j == 0) {
+                        dg = 0.0;
+                    } else {
+                        if (i > 0 && j > 0) dg = D[(size_t)(i - 1) * bw + t];      /* (i-1, j-1) */
+                        if (i > 0 && t + 1 < bw) up = D[(size_t)(i - 1) * bw + t + 1]; /* (i-1, j) */
+                        if (j > 0 && t > 0) lf = D[(size_t)i * bw + t - 1];        /* (i, j-1) */
+                    }
+                    double mn;
+                    if (dg <= up && dg <= lf) {
+                        mn = dg;
+                        dr = 0;
+                    } else if (up <= lf) {
+                        mn = up;
+                        dr = 1;
+                    } else {
+                        mn = lf;
+                        dr = 2;
+                    }
+                    const double d = zq[i] - z[j];
+                    v = (metric ? d * d : fabs(d)) + mn;
+                }
+                D[(size_t)i * bw + t] = v;
+                dir[(size_t)i * bw + t] = dr;
+            }
+        }
+        const double Dm = D[(size_t)(m - 1) * bw + w];
+        double scale = g[p];
+        if (sqrt_dist) scale = Dm > 0.0 ? scale * 0.5 / sqrt(Dm) : 0.0;
+        if (dist) dist[p] = sqrt_dist ? sqrt(Dm) : Dm;
+        for (int i = 0; i < m; ++i) row[i] = 0.0;
+        int i = m - 1, j = m - 1;
+        for (;;) {
+            const double d = zq[i] - z[j];
+            row[i] += metric ? 2.0 * d : (d > 0 ? 1.0 : (d < 0 ? -1.0 : 0.0));
+            if (i == 0 && j == 0) break;
+            const uint8_t dr = dir[(size_t)i * bw + (j - i + w)];
+            if (dr == 0) {
+                --i;
+                --j;
+            } else if (dr == 1) {
+                --i;
+            } else {
+                --j;
+            }
+        }
+        for (int r = 0; r < m; ++r) G[r] += scale * row[r];
+    }
+    if (q_raw) {
+        double mu = 0.0, var = 0.0, mg = 0.0, mgz = 0.0;
+        for (int i = 0; i < m; ++i) mu += q_raw[i];
+        mu /= m;
+        for (int i = 0; i < m; ++i) var += (q_raw[i] - mu) * (q_raw[i] - mu);
+        const double sig = sqrt(var / m);
+        for (int i = 0; i < m; ++i) {
+            mg += G[i];
+            mgz += G[i] * zq[i];
+        }
+        mg /= m;
+        mgz /= m;
+        for (int i = 0; i < m; ++i) grad[i] = (G[i] - mg - zq[i] * mgz) / sig;
+    } else {
+        for (int i = 0; i < m; ++i) grad[i] = G[i];
+    }
+    free(G);
+    free(D);
+    free(D32);
+    free(dir);
+    free(z);
+    free(row);
+}
+
+/* ------------------------------------------------------------------------ */
 /* ldtw.cuh:74-127 z_serial_ldtw; INFTY = 1E10 (ldtw.cuh:8). */
 #define LDTW_INFTY ((float)1E10)
 

@@ -28,6 +28,9 @@
  *                           is the path's cost) and LDTW >= cDTW.
  *   or_ed_f64               numpy direct sum; UCR_ED_BIN location (golden).
  *   or_topk_greedy          brute-force enumeration on tiny inputs.
+ *   or_cdtw_grad_f64        FP64 central differences of or_cdtw_f64 (through
+ *                           or_znormalize_f64 for the raw-query form); W = 0
+ *                           closed form 2(q - z); path cost equals or_cdtw_f64.
  *   or_ucr_dtw / or_ucr_ed  UCR_DTW_BIN / UCR_ED_BIN compiled from the
  *                           reference sources (oracle/_ref), location golden.
  */
@@ -85,6 +88,35 @@ void or_cdtw(const float* zq, const float* s, const float* avg, const float* sd,
 void or_cdtw_f64(const double* zq, const float* s, const double* avg,
                  const double* sd, int32_t m, const int32_t* idx, int64_t count,
                  int32_t w, int32_t metric, double* out);
+
+/* Gradient of  sum_p g[p] * dist_p  with respect to the query, FP64
+ * throughout.  dist_p is the banded cDTW of the z-normalised query zq against
+ * window idx[p] (or p) -- the D[M][M] of or_cdtw_f64 (cdtw.cuh:23-88), or its
+ * sqrt when sqrt_dist (the reported distance, cdtw.cuh:507).  The DP keeps,
+ * per cell, which predecessor gave the minimum (ties: diagonal if it is not
+ * above either other, then (i-1, j) if not above (i, j-1), else (i, j-1));
+ * the path is walked back from (M-1, M-1) and each cell (i, j) on it adds
+ * d cost / d zq_i = 2 (zq_i - z_j) (metric 1) or sign(zq_i - z_j) (metric 0).
+ * sqrt_dist with D = 0 contributes 0 (the sqrt is not differentiable there).
+ * q_raw NULL: grad = d/d zq.  Otherwise zq must be or_znormalize_f64(q_raw)
+ * and grad is carried through the z-normalisation (population sigma):
+ * dL/dq = (G - mean(G) - zq * mean(G * zq)) / sigma.  dist may be NULL. */
+void or_cdtw_grad_f64(const double* zq, const float* s, const double* avg,
+                      const double* sd, int32_t m, const int32_t* idx,
+                      int64_t count, int32_t w, int32_t metric,
+                      int32_t sqrt_dist, const double* g, const double* q_raw,
+                      double* dist, double* grad);
+
+/* or_cdtw_grad_f64 (path32 = 0), or with path32 = 1 the path of the FP32 DP
+ * of or_cdtw (fused = 1; z = (s - avg) / sd and d = q - z in FP32, zq and the
+ * statistics taken as FP32), i.e. the path an FP32 implementation follows;
+ * the gradient terms are still summed in FP64.  dist then gets the FP32
+ * distance. */
+void or_cdtw_grad(const double* zq, const float* s, const double* avg,
+                  const double* sd, int32_t m, const int32_t* idx,
+                  int64_t count, int32_t w, int32_t metric,
+                  int32_t sqrt_dist, int32_t path32, const double* g,
+                  const double* q_raw, double* dist, double* grad);
 
 /* ldtw.cuh:74-127 z_serial_ldtw, for windows idx[0..count) (idx NULL: 0..).
  * moves (optional, count*(2m-2) bytes): 0 = diagonal, 1 = down, 2 = right,

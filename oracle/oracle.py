@@ -47,6 +47,10 @@ def lib() -> C.CDLL:
                                C.c_int32, _F]),
             "or_cdtw_f64": (None, [_D, _F, _D, _D, C.c_int32, _I, C.c_int64, C.c_int32, C.c_int32,
                                    _D]),
+            "or_cdtw_grad_f64": (None, [_D, _F, _D, _D, C.c_int32, _I, C.c_int64, C.c_int32,
+                                        C.c_int32, C.c_int32, _D, _D, _D, _D]),
+            "or_cdtw_grad": (None, [_D, _F, _D, _D, C.c_int32, _I, C.c_int64, C.c_int32,
+                                    C.c_int32, C.c_int32, C.c_int32, _D, _D, _D, _D]),
             "or_ldtw": (None, [_F, _F, _F, _F, C.c_int32, _I, C.c_int64, C.c_int32, C.c_int32, _F,
                                _U8]),
             "or_ed_f64": (None, [_D, _F, C.c_int64, C.c_int32, _D, _D, _D]),
@@ -163,6 +167,23 @@ def cdtw_f64(zq, s, avg64, sd64, w, idx=None, count=None, metric=1):
     lib().or_cdtw_f64(_d(zq), _f(s), _d(avg64), _d(sd64), zq.size, ip, n, int(w), int(metric),
                       _d(out))
     return out
+
+
+def cdtw_grad_f64(zq64, s, avg64, sd64, w, idx, g, metric=1, sqrt_dist=1, q_raw=None,
+                  path32=False):
+    """FP64 query gradient of sum_p g[p] * dist_p (or_cdtw_grad; path32: along
+    the optimal path of the FP32 DP).  Returns (grad (M,), dist (count,))."""
+    zq64, s, avg64, sd64 = _c64(zq64), _c32(s), _c64(avg64), _c64(sd64)
+    idx, ip = _idx(idx)
+    g = _c64(g)
+    m = zq64.size
+    grad = np.empty(m, np.float64)
+    dist = np.empty(idx.size, np.float64)
+    qr = None if q_raw is None else _c64(q_raw)
+    lib().or_cdtw_grad(_d(zq64), _f(s), _d(avg64), _d(sd64), m, ip, idx.size, int(w),
+                       int(metric), int(sqrt_dist), int(bool(path32)), _d(g),
+                       None if qr is None else _d(qr), _d(dist), _d(grad))
+    return grad, dist
 
 
 def ldtw(zq, s, avg, sd, w, idx=None, count=None, metric=1, want_moves=False):

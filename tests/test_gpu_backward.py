@@ -1,0 +1,172 @@
+"""Query gradient of cDTW on the GPU (mdtw_cdtw_backward) against the FP64
+oracle gradient and FP64 central differences.
+
+No reference counterpart: the bar is the FP64 CPU gradient (or_cdtw_grad_f64,
+itself pinned by central differences in test_oracle_grad.py) and FP64
+central differences with eps = 1e-3, both to 1e-4 relative (2-norm over the
+query), and the recomputed distances bit-identical to the forward profile
+(mdtw_cdtw).
+"""
+import numpy as np
+import pytest
+import torch
+
+from cudadtw_amd import METRIC_L1, METRIC_L2, synth
+from cudadtw_amd.autograd import cdtw_at, topk_cdtw
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-4
+
+
+def dev(a, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def setup(engine, m, w, seed, n=1 << 14, count=8):
+    rng = np.random.default_rng(seed)
+    s = synth.ecg_like(n, seed)
+    q = np.cumsum(rng.standard_normal(m)).astype(np.float32)
+    zq = O.znormalize_f32(q)
+    engine.set_query(zq, w)
+    ds = dev(s)
+    avg, sd = engine.avg_std(ds, m)
+    idx = np.sort(rng.choice(n - m + 1, size=count, replace=False)).astype(np.int32)
+    g = rng.uniform(0.25, 2.0, size=count).astype(np.float32)
+    return s, q, zq, ds, avg, sd, idx, g
+
+
+def rel_err(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("m,w", [(16, 0), (64, 6), (128, 13), (200, 20), (256, 51), (300, 90),
+                                 (160, 159), (512, 51)])
+@pytest.mark.parametrize("metric", [METRIC_L2, METRIC_L1])
+def test_backward_vs_oracle_gradient(engine, m, w, metric):
+    s, q, zq, ds, avg, sd, idx, g = setup(engine, m, w, 1000 + m + w)
+    sqrt_dist = metric == METRIC_L2
+    gq, dist = engine.cdtw_backward(ds, avg, sd, dev(idx, torch.int32), dev(g), w, metric,
+                                    sqrt_dist=sqrt_dist, want_dist=True)
+    prof = host(engine.gpu_cdtw(ds, avg, sd, w, indices=dev(idx, torch.int32), metric=metric))
+    want_d = np.sqrt(prof) if sqrt_dist else prof
+    np.testing.assert_array_equal(host(dist), want_d.astype(np.float32))
+    a, d = host(avg).astype(np.float64), host(sd).astype(np.float64)
+    # Along the path of the FP32 DP (the path this kernel must follow): the
+    # only difference left is FP32 vs FP64 summation of the path terms.
+    o32, d32 = O.cdtw_grad_f64(zq.astype(np.float64), s, a, d, w, idx, g.astype(np.float64),
+                               metric=metric, sqrt_dist=int(sqrt_dist), path32=True)
+    np.testing.assert_array_equal(host(dist), d32.astype(np.float32))
+    err32 = rel_err(host(gq), o32)
+    og, _ = O.cdtw_grad_f64(zq.astype(np.float64), s, a, d, w, idx, g.astype(np.float64),
+                            metric=metric, sqrt_dist=int(sqrt_dist))
+    err = rel_err(host(gq), og)
+    print(f"m={m} w={w} metric={metric}: rel err vs FP32-path gradient {err32:.2e}, "
+          f"vs FP64 gradient {err:.2e}")
+    assert err32 < 1e-5
+    if metric == METRIC_L2:
+        assert err < REL
+    # L1 local costs |q - z| make exact and near ties between warping paths
+    # common; where the FP32 and FP64 DPs break such a tie differently the
+    # gradients differ (both are valid subgradients), so only the FP32-path
+    # comparison is a parity check for L1.
+
+
+@pytest.mark.parametrize("m,w", [(64, 6), (128, 13), (256, 26)])
+def test_backward_raw_query_vs_central_differences(engine, m, w):
+    s, q, zq, ds, avg, sd, idx, g = setup(engine, m, w, 77 + m)
+    gq, _ = engine.cdtw_backward(ds, avg, sd, dev(idx, torch.int32), dev(g), w,
+                                 query_raw=dev(q))
+    gq = host(gq)
+    a, d = host(avg).astype(np.float64), host(sd).astype(np.float64)
+    q64 = q.astype(np.float64)
+    og, _ = O.cdtw_grad_f64(O.znormalize_f64(q64.copy()), s, a, d, w, idx, g.astype(np.float64),
+                            q_raw=q64)
+    assert rel_err(gq, og) < REL
+
+    def loss(qq):
+        z = O.znormalize_f64(qq.copy())
+        return float(np.sum(g * np.sqrt(O.cdtw_f64(z, s, a, d, w, idx=idx))))
+
+    # Central differences with eps = 1e-3, and with eps / 2 to find the
+    # coordinates where a change of optimal warping path lies within +-eps
+    # (cDTW is piecewise smooth; across such a kink a difference quotient is
+    # no derivative).  On smooth coordinates the two agree to O(eps^2).
+    def central(eps):
+        fd = np.empty(m)
+        for i in range(m):
+            e = np.zeros(m)
+            e[i] = eps
+            fd[i] = (loss(q64 + e) - loss(q64 - e)) / (2 * eps)
+        return fd
+
+    fd1, fd2 = central(1e-3), central(5e-4)
+    smooth = np.abs(fd1 - fd2) <= 1e-5 * np.abs(fd1).max()
+    err = rel_err(gq[smooth], fd1[smooth])
+    print(f"m={m} w={w}: rel err vs FP64 central differences {err:.2e} on "
+          f"{smooth.sum()}/{m} smooth coordinates")
+    assert smooth.mean() >= 0.75
+    assert err < REL
+
+
+def test_backward_empty_and_deterministic(engine):
+    s, q, zq, ds, avg, sd, idx, g = setup(engine, 96, 9, 5, count=64)
+    di, dg = dev(idx, torch.int32), dev(g)
+    g1, _ = engine.cdtw_backward(ds, avg, sd, di, dg, 9)
+    g2, _ = engine.cdtw_backward(ds, avg, sd, di, dg, 9)
+    np.testing.assert_array_equal(host(g1), host(g2))
+    g0, _ = engine.cdtw_backward(ds, avg, sd, di[:0], dg[:0], 9)
+    assert np.all(host(g0) == 0.0)
+
+
+def test_autograd_fixed_windows(engine):
+    m, w = 128, 13
+    s, q, zq, ds, avg, sd, idx, g = setup(engine, m, w, 31)
+    qt = torch.tensor(q, dtype=torch.float32, requires_grad=True)
+    dist = cdtw_at(qt, engine, ds, avg, sd, dev(idx, torch.int32), w)
+    loss = (dist * dev(g)).sum()
+    loss.backward()
+    a, d = host(avg).astype(np.float64), host(sd).astype(np.float64)
+    q64 = q.astype(np.float64)
+    og, _ = O.cdtw_grad_f64(O.znormalize_f64(q64.copy()), s, a, d, w, idx, g.astype(np.float64),
+                            q_raw=q64)
+    assert qt.grad is not None and qt.grad.shape == (m,)
+    assert rel_err(qt.grad.numpy(), og) < REL
+
+
+def test_autograd_topk_and_template_refinement(engine):
+    # The loss is the sum of the top-8 distances; gradient steps on the
+    # template (a DBA-style refinement) must lower it.
+    m, w, k = 128, 13, 8
+    s = synth.ecg_like(1 << 16, 8)
+    ds = dev(s)
+    avg, sd = engine.avg_std(ds, m)
+    rng = np.random.default_rng(8)
+    q0 = s[5000:5000 + m] + rng.normal(0, 0.3, m).astype(np.float32)
+    qt = torch.tensor(q0, dtype=torch.float32, requires_grad=True)
+    dist, pos = topk_cdtw(qt, engine, ds, avg, sd, w, k, m // 2)
+    assert dist.numel() == k
+    loss0 = float(dist.detach().sum())
+    dist.sum().backward()
+    # check against the oracle gradient at the found positions
+    a, d = host(avg).astype(np.float64), host(sd).astype(np.float64)
+    q64 = q0.astype(np.float64)
+    og, _ = O.cdtw_grad_f64(O.znormalize_f64(q64.copy()), s, a, d, w, host(pos),
+                            np.ones(k), q_raw=q64)
+    assert rel_err(qt.grad.numpy(), og) < REL
+    opt = torch.optim.SGD([qt], lr=0.05)
+    losses = [loss0]
+    for _ in range(20):
+        opt.zero_grad()
+        dist, _ = topk_cdtw(qt, engine, ds, avg, sd, w, k, m // 2)
+        dist.sum().backward()
+        opt.step()
+        losses.append(float(dist.detach().sum()))
+    print("refinement losses", [round(x, 4) for x in losses[::5]])
+    assert losses[-1] < 0.9 * losses[0]
