@@ -35,6 +35,18 @@ struct WideScalars {
 
 constexpr int kWideCheck = 32;  // steps between abandon checks
 
+// Whole-wave shifts by one lane through DPP (wave_shr:1 / wave_shl:1, GFX9
+// family): register-only, no LDS round trip.  The lane with no source gets
+// `fill`.
+__device__ __forceinline__ float lane_from_below(float v, float fill) {  // lane l <- l-1
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v),
+                                                      0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_from_above(float v, float fill) {  // lane l <- l+1
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v),
+                                                      0x130, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ float wave_min_f32(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
@@ -64,14 +76,18 @@ __device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
     }
     int jb = 0;
     float zb = lane < m ? (x[lane] - mu) / sg : -kInf;  // z[jb + lane]
+    // Query values enter at the top slot, q[(k - W) / 2 + 64C] on even steps;
+    // they are fetched 64 at a time into qb (q[ib + lane]) and taken with
+    // readlane, so no step waits on a load.
+    int ib = -kWave;
+    float qb = kInf;
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     const int K = 2 * m - 1;
     for (int k = 0; k < K; ++k) {
         const int par = (k + W) & 1;
         float nw[C];
         if (par == 0) {
-            float in = __shfl_up(A1[C - 1], 1, kWave);
-            if (lane == 0) in = kInf;
+            const float in = lane_from_below(A1[C - 1], kInf);
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const float up = c > 0 ? A1[c - 1] : in;
@@ -81,8 +97,7 @@ __device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
                 nw[c] = fmaxf(v, pad0[c]);
             }
         } else {
-            float in = __shfl_down(A1[0], 1, kWave);
-            if (lane == kWave - 1) in = kInf;
+            const float in = lane_from_above(A1[0], kInf);
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 const float lf = c + 1 < C ? A1[c + 1] : in;
@@ -99,15 +114,22 @@ __device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
         }
         if (par == 0) {
             // i advances: slot p takes slot p+1's query value.
-            const float in = __shfl_down(Qv[0], 1, kWave);
+            const float in = lane_from_above(Qv[0], kInf);
 #pragma unroll
             for (int c = 0; c + 1 < C; ++c) Qv[c] = Qv[c + 1];
             const int iq = (k - W) / 2 + kWave * C;
-            const float fresh = (iq >= 0 && iq < m) ? q[iq] : kInf;
+            float fresh = kInf;
+            if (iq >= 0 && iq < m) {
+                if (iq >= ib + kWave) {
+                    ib = iq & ~(kWave - 1);
+                    qb = ib + lane < m ? q[ib + lane] : kInf;
+                }
+                fresh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), iq - ib));
+            }
             Qv[C - 1] = lane == kWave - 1 ? fresh : in;
         } else {
             // j advances: slot p takes slot p-1's stream value.
-            const float in = __shfl_up(Zv[C - 1], 1, kWave);
+            const float in = lane_from_below(Zv[C - 1], -kInf);
 #pragma unroll
             for (int c = C - 1; c > 0; --c) Zv[c] = Zv[c - 1];
             const int iz = (k + W + 1) / 2;
