@@ -170,25 +170,82 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-// Search cascade over windows 0..L-1, wave-strided: each wave takes 64
-// consecutive windows per iteration.  Survivors go to cand (order not
+// Search cascade over windows wbase..wbase+L-1, wave-strided: each wave takes
+// 64 consecutive windows per iteration.  Survivors go to cand (order not
 // significant: the search result does not depend on it).
+//
+// The 64 windows of a wave share all but 63 of their samples, so LB_Keogh
+// reads the stream from a per-wave ring in LDS instead of one vector load
+// per lane and element: lane l at step i needs sample wb + l + i.  The ring
+// holds 64-sample chunk c at slots (64c mod 128) + l and, duplicated, at
+// + 128, so the steps 64c .. 64c + 63 (which read chunks c and c + 1) read
+// slots (64c mod 128) + l + u for u < 64 with no wrap: one ds_read with an
+// immediate offset per step.  After those steps chunk c + 2 replaces chunk
+// c.  The query envelope of those 64 steps is loaded lane-distributed
+// (lane l holds lo/up[64c + l]) and broadcast per step with readlane, so the
+// inner loop issues no global or scalar loads.  A wave reads only its own
+// ring and LDS operations of one wave complete in order; a wave-level fence
+// orders each refill before the reads that follow.
+constexpr int kCascadeWaves = 4;
+constexpr int kRing = 256;
+// Survivors are staged per wave in LDS and appended kCandBuf - 64 or more at
+// a time: one atomicAdd on the shared candidate counter per flush instead of
+// one per 64 windows (same-address atomics serialise in L2).
+constexpr int kCandBuf = 512;
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 template <bool L2>
-__global__ __launch_bounds__(256) void cascade_kernel(
+__device__ __forceinline__ float keogh_step(float xv, float mu, float rs, float l, float u,
+                                            float acc) {
+    const float z = (xv - mu) * rs;
+    const float e = z - __builtin_amdgcn_fmed3f(z, l, u);
+    return L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+}
+
+template <bool L2>
+__global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
     const float* __restrict__ q, const float* __restrict__ lo, const float* __restrict__ up,
-    int32_t m, int64_t L, bool use_best, Ctrl* ctrl, int32_t* __restrict__ cand) {
+    int32_t m, int64_t wbase, int64_t L, bool use_best, Ctrl* ctrl,
+    int32_t* __restrict__ cand) {
+    __shared__ float ring_all[kCascadeWaves][kRing];
+    __shared__ int32_t cbuf_all[kCascadeWaves][kCandBuf];
     const int lane = threadIdx.x & (kWave - 1);
+    float* ring = ring_all[threadIdx.x / kWave];
+    int32_t* cbuf = cbuf_all[threadIdx.x / kWave];
+    int32_t nbuf = 0;
+    auto flush = [&]() {
+        wave_lds_fence();
+        unsigned long long basec = 0;
+        if (lane == 0)
+            basec = atomicAdd(&ctrl->count, static_cast<unsigned long long>(nbuf));
+        basec = __shfl(basec, 0, kWave);
+        for (int32_t t = lane; t < nbuf; t += kWave) cand[basec + t] = cbuf[t];
+        wave_lds_fence();
+        nbuf = 0;
+    };
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
     const float thr = use_best ? load_best_dist(ctrl) : load_thr(ctrl);
     const float lim = prune_limit(thr, m, L2);
+    // Last stream sample any window of this range reads.
+    const int64_t last = wbase + L - 1 + m - 1;
     unsigned long long n_kim = 0, n_keogh = 0;
 
     for (int64_t base = wave * kWave; base < L; base += nwaves * kWave) {
-        const int64_t w = base + lane;
-        const bool valid = w < L;
-        const int64_t wc = valid ? w : L - 1;
+        const bool valid = base + lane < L;
+        const int64_t wb = wbase + base;  // first window of the wave
+        const int64_t w = wb + lane;
+        const int64_t wc = valid ? w : wbase + L - 1;
         const float* x = s + wc;
         const float mu = avg[wc], sg = sd[wc];
         const float rs = 1.0f / sg;
@@ -196,37 +253,53 @@ __global__ __launch_bounds__(256) void cascade_kernel(
         bool alive = valid && !(kim > lim);
         n_kim += __popcll(__ballot(valid && !alive));
         if (!__any(alive)) continue;
+
+        auto fill = [&](int32_t c) {  // samples wb + 64c .. wb + 64c + 63
+            const int64_t gi = wb + static_cast<int64_t>(c) * kWave + lane;
+            const float v = s[gi < last ? gi : last];
+            const int slot = (c & 1) * kWave + lane;
+            ring[slot] = v;
+            ring[slot + 2 * kWave] = v;
+        };
+        fill(0);
+        fill(1);
+        wave_lds_fence();
         float acc = 0.0f;
-        int32_t i = 0;
         bool gone = false;
-        for (; i + 16 <= m; i += 16) {
+        for (int32_t i0 = 0; i0 < m && !gone; i0 += kWave) {
+            const int32_t ie = lane + i0;
+            const float lv = ie < m ? lo[ie] : 0.0f;
+            const float uv = ie < m ? up[ie] : 0.0f;
+            const float* rp = ring + (i0 & kWave) + lane;  // (64c mod 128) + lane
+            const int32_t nstep = m - i0 < kWave ? m - i0 : kWave;
+            int32_t u = 0;
+            for (; u + 16 <= nstep; u += 16) {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const float z = (x[i + u] - mu) * rs;
-                const float e = z - __builtin_amdgcn_fmed3f(z, lo[i + u], up[i + u]);
-                acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+                for (int t = 0; t < 16; ++t)
+                    acc = keogh_step<L2>(rp[u + t], mu, rs, bcast(lv, u + t), bcast(uv, u + t),
+                                         acc);
+                if (!__any(alive && !(acc > lim))) {
+                    gone = true;
+                    break;
+                }
             }
-            if (!__any(alive && !(acc > lim))) {
-                gone = true;
-                break;
+            if (gone) break;
+            for (; u < nstep; ++u) acc = keogh_step<L2>(rp[u], mu, rs, bcast(lv, u), bcast(uv, u), acc);
+            if (i0 + kWave < m) {
+                fill(i0 / kWave + 2);  // replaces the chunk just consumed
+                wave_lds_fence();
             }
         }
-        for (; !gone && i < m; ++i) {
-            const float z = (x[i] - mu) * rs;
-            const float e = z - __builtin_amdgcn_fmed3f(z, lo[i], up[i]);
-            acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
-        }
-        const bool keep = alive && !(acc > lim);
+        const bool keep = alive && !gone && !(acc > lim);
         const unsigned long long km = __ballot(keep);
         n_keogh += __popcll(__ballot(alive && !keep));
         if (km) {
-            unsigned long long basec = 0;
-            if (lane == 0)
-                basec = atomicAdd(&ctrl->count, static_cast<unsigned long long>(__popcll(km)));
-            basec = __shfl(basec, 0, kWave);
-            if (keep) cand[basec + lane_rank(km)] = static_cast<int32_t>(w);
+            if (keep) cbuf[nbuf + lane_rank(km)] = static_cast<int32_t>(w);
+            nbuf += __popcll(km);
+            if (nbuf > kCandBuf - kWave) flush();
         }
     }
+    if (nbuf) flush();
     if (lane == 0) {
         if (n_kim) atomicAdd(&ctrl->kim, n_kim);
         if (n_keogh) atomicAdd(&ctrl->keogh, n_keogh);
@@ -273,8 +346,8 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg, const floa
 }
 
 int launch_cascade(hipStream_t st, const float* s, const float* avg, const float* sd,
-                   const float* q, const float* lo, const float* up, int32_t m, int64_t L,
-                   int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand) {
+                   const float* q, const float* lo, const float* up, int32_t m, int64_t base,
+                   int64_t L, int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand) {
     if (L <= 0) return MDTW_OK;
     // Enough waves to cover the chip several times over; each wave strides.
     const int64_t waves_needed = (L + kWave - 1) / kWave;
@@ -282,10 +355,10 @@ int launch_cascade(hipStream_t st, const float* s, const float* avg, const float
     const unsigned nb = static_cast<unsigned>(nb64);
     if (metric)
         hipLaunchKernelGGL(cascade_kernel<true>, dim3(nb), dim3(256), 0, st, s, avg, sd, q, lo,
-                           up, m, L, use_best, ctrl, cand);
+                           up, m, base, L, use_best, ctrl, cand);
     else
         hipLaunchKernelGGL(cascade_kernel<false>, dim3(nb), dim3(256), 0, st, s, avg, sd, q,
-                           lo, up, m, L, use_best, ctrl, cand);
+                           lo, up, m, base, L, use_best, ctrl, cand);
     return check_launch();
 }
 

@@ -188,9 +188,11 @@ __global__ __launch_bounds__(64) void wide_kernel(
             if (a.mode == kModeProfile || a.mode == kModeSeed) out[g] = d;
             if ((a.mode == kModeBest || a.mode == kModeSeed) && alive)
                 atomicMin(&ctrl->best, pack_key(d, static_cast<uint32_t>(w)));
+            if (a.mode == kModeSeed) note_bucket(ctrl->bmin, ctrl->bucket_len, w, d);
             if (a.mode == kModeCollect && alive && d <= load_thr(ctrl)) {
                 const unsigned long long r = atomicAdd(&ctrl->nres, 1ull);
                 res[r] = pack_key(d, static_cast<uint32_t>(w));
+                note_bucket(ctrl->bmin, ctrl->bucket_len, w, d);
             }
         }
         n_dp += 1;

@@ -199,6 +199,8 @@ __global__ __launch_bounds__(256) void band_kernel(
         count = static_cast<int64_t>(
             __hip_atomic_load(&ctrl->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     unsigned long long n_dp = 0, n_ab = 0;
+    uint32_t* const bmin = ctrl->bmin;
+    const long long bucket_len = ctrl->bucket_len;
     // persistent: waves take 64 windows at a time from an atomic cursor;
     // otherwise a static grid stride (every wave the same amount of work).
     const int64_t step = static_cast<int64_t>(gridDim.x) * kBandWaves * kWave;
@@ -219,8 +221,10 @@ __global__ __launch_bounds__(256) void band_kernel(
             unsigned long long key = alive ? pack_key(d, static_cast<uint32_t>(w)) : ~0ull;
             key = wave_min_u64(key);
             if (lane == 0 && key != ~0ull) atomicMin(&ctrl->best, key);
+            if (a.mode == kModeSeed && valid) note_bucket(bmin, bucket_len, w, d);
         } else if (a.mode == kModeCollect) {
             const bool keep = alive && d <= load_thr(ctrl);
+            if (keep) note_bucket(bmin, bucket_len, w, d);
             const unsigned long long km = __ballot(keep);
             if (km) {
                 unsigned long long rb = 0;

@@ -73,8 +73,10 @@ __global__ __launch_bounds__(256) void ed_kernel(
             unsigned long long key = alive ? pack_key(acc, static_cast<uint32_t>(w)) : ~0ull;
             key = wave_min_u64(key);
             if (lane == 0 && key != ~0ull) atomicMin(&ctrl->best, key);
+            if (a.mode == kModeSeed && valid) note_bucket(ctrl->bmin, ctrl->bucket_len, w, acc);
         } else if (a.mode == kModeCollect) {
             const bool keep = alive && acc <= thr;
+            if (keep) note_bucket(ctrl->bmin, ctrl->bucket_len, w, acc);
             const unsigned long long km = __ballot(keep);
             if (km) {
                 unsigned long long rb = 0;

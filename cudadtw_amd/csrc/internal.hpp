@@ -35,8 +35,14 @@ struct Ctrl {
     unsigned long long keogh;    // pruned by LB_Keogh
     unsigned long long dp;       // DP runs started
     unsigned long long abandoned;// DP runs stopped early
-    float thr;                   // fixed threshold (top-k), squared distance
-    float pad[7];
+    float thr;                   // top-k threshold (squared distance); only falls
+    uint32_t unused;
+    // Top-k: per-bucket minimum distance (float bits) of the windows whose
+    // distance is known exactly, bucket = window / bucket_len; NULL when not
+    // kept.  See launch_tighten.
+    uint32_t* bmin;
+    long long bucket_len;
+    long long pad[2];
 };
 
 }  // namespace mdtw
@@ -170,12 +176,13 @@ int launch_lb_kim(hipStream_t st, const float* s, const float* avg,
 int launch_lb_keogh(hipStream_t st, const float* s, const float* avg,
                     const float* sd, const float* lo, const float* up, int32_t m,
                     const int32_t* idx, int64_t count, int32_t metric, float* out);
-// Search cascade: LB_Kim then LB_Keogh on every window 0..L-1; survivors
-// (bound not above the limit of ctrl->best, or of ctrl->thr) go to cand.
+// Search cascade: LB_Kim then LB_Keogh on windows base..base+count-1;
+// survivors (bound not above the limit of ctrl->best, or of ctrl->thr) are
+// appended to cand (ctrl->count).
 int launch_cascade(hipStream_t st, const float* s, const float* avg,
                    const float* sd, const float* q, const float* lo,
-                   const float* up, int32_t m, int64_t L, int32_t metric,
-                   bool use_best, Ctrl* ctrl, int32_t* cand);
+                   const float* up, int32_t m, int64_t base, int64_t count,
+                   int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand);
 // cdtw.hip: band kernel (qx = padded query) or, for wide bands, the
 // anti-diagonal kernel (q_plain).
 int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain,
@@ -188,7 +195,21 @@ int launch_ed(hipStream_t st, int cus, const BandArgs& a);
 // best_init: kBestNone for searches (key of +inf, so load_best_dist is +inf
 // until a distance lands), ~0ull for the signed-order argmin.
 constexpr unsigned long long kBestNone = (0x7f800000ull << 32) | 0xffffffffull;
-int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init);
+int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init,
+                      uint32_t* bmin = nullptr, int64_t bucket_len = 1);
+// Between search rounds: candidate count and work cursor back to 0.
+int launch_round_reset(hipStream_t st, Ctrl* ctrl);
+// Top-k bound from the bucket minima (nb buckets, +inf where unknown):
+// ctrl->thr = min(ctrl->thr, T) with T >= the k-th smallest bucket minimum
+// (split: over even and over odd buckets separately, T = the smaller).  With
+// buckets of 2*excl windows, windows in distinct buckets of one parity are
+// more than 2*excl apart, so one greedy pick (which excludes |i - j| < excl)
+// removes at most one of them; the k-th pick is then no worse than the k-th
+// smallest of either parity class.  scratch: tighten_ws_words() words.
+size_t tighten_ws_words();
+int launch_tighten(hipStream_t st, const uint32_t* bmin, int64_t nb, int32_t k, bool split,
+                   uint32_t* scratch, Ctrl* ctrl);
+int launch_fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v);
 int launch_thr_from_best(hipStream_t st, Ctrl* ctrl);
 int launch_argmin(hipStream_t st, const float* v, int64_t count, Ctrl* ctrl);
 size_t threshold_ws_ints(int64_t count);
@@ -228,6 +249,14 @@ __device__ __forceinline__ float load_best_dist(const Ctrl* c) {
 }
 __device__ __forceinline__ float load_thr(const Ctrl* c) {
     return __hip_atomic_load(&c->thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exact distance d of window w is known: lower its bucket's minimum.
+__device__ __forceinline__ void note_bucket(uint32_t* bmin, long long bucket_len, int64_t w,
+                                            float d) {
+    if (bmin)
+        atomicMin(bmin + static_cast<uint32_t>(w) / static_cast<uint32_t>(bucket_len),
+                  __float_as_uint(d));
 }
 
 // Pruning limit that keeps pruning exact.  A computed lower bound can exceed

@@ -120,8 +120,10 @@ __global__ __launch_bounds__(64) void ldtw_kernel(
             unsigned long long key = alive ? pack_key(d, static_cast<uint32_t>(w)) : ~0ull;
             key = wave_min_u64(key);
             if (lane == 0 && key != ~0ull) atomicMin(&ctrl->best, key);
+            if (a.mode == kModeSeed && valid) note_bucket(ctrl->bmin, ctrl->bucket_len, w, d);
         } else if (a.mode == kModeCollect) {
             const bool keep = alive && d <= load_thr(ctrl);
+            if (keep) note_bucket(ctrl->bmin, ctrl->bucket_len, w, d);
             const unsigned long long km = __ballot(keep);
             if (km) {
                 unsigned long long rb = 0;

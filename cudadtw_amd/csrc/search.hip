@@ -10,20 +10,28 @@
 //   2. seed: exact distances of up to 16384 evenly strided windows, which
 //      sets the bound (k = 1: best seed; k > 1: k-th smallest seed, valid
 //      because the seeds are >= 2*excl apart, so no pick can exclude two),
-//   3. cascade: LB_Kim then LB_Keogh over every window, survivors appended
-//      to a candidate list,
-//   4. persistent DP over the candidates with early abandoning,
-//      k = 1: atomicMin of (distance, index) keys (lowest index wins ties);
-//      k > 1: every candidate at or below the bound is kept as a key,
-//   5. k > 1: the kept keys are sorted and picked greedily with the
+//   3. rounds over window ranges (one for k = 1; for k > 1 the first L/16
+//      windows, then the rest -- the reference refines its bound over
+//      doubling chunks in LB_Kim order, cdtw.cuh:551-585; here stream order,
+//      so no sort is needed), each:
+//      cascade (LB_Kim then LB_Keogh, survivors appended to a candidate
+//      list), then a persistent DP over the candidates with early
+//      abandoning; k = 1: atomicMin of (distance, index) keys (lowest index
+//      wins ties), which the next round's cascade prunes against; k > 1:
+//      every candidate at or below the bound is kept as a key, its distance
+//      lowers its bucket's minimum, and after the round the bound is
+//      tightened from the bucket minima (launch_tighten),
+//   4. k > 1: the kept keys are sorted and picked greedily with the
 //      exclusion zone on the host (they are few).
 // Pruning never changes the answer: windows are dropped only when a bound
 // strictly exceeds prune_limit(threshold) (rounding-aware), and a DP run is
 // abandoned only when its running column minimum strictly exceeds the
-// threshold.
+// threshold; the threshold only falls and never below the k-th pick's
+// distance, and every window at or below the final threshold is kept.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "cdtw_band.hpp"
 
@@ -48,6 +56,23 @@ SeedPlan plan_seeds(int64_t L, int32_t k, int64_t excl) {
         count = (L - 1) / stride + 1;
     }
     return {count, stride};
+}
+
+// Window ranges of the search rounds.  k = 1: one pass (the DP lowers the
+// best distance as it goes, and on repetitive streams an early, tighter
+// bound barely changes what LB_Keogh prunes).  k > 1: the first L/16
+// windows, then the rest, so the bound from the first range's bucket minima
+// (far below the k-th seed) prunes the rest.
+struct Round {
+    int64_t base;
+    int64_t count;
+};
+constexpr int64_t kRoundMin = int64_t(1) << 16;
+
+std::vector<Round> plan_rounds(int64_t L, bool topk) {
+    const int64_t first = std::max<int64_t>(kRoundMin, L / 16);
+    if (!topk || first >= L) return {{0, L}};
+    return {{0, first}, {first, L - first}};
 }
 
 int hip_check(hipError_t err) {
@@ -113,20 +138,31 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     const SeedPlan sp = plan_seeds(L, k, excl);
     const bool own_stats = d_avg == nullptr;
     const bool topk = k > 1;
+    // Top-k bound buckets: 2*excl windows (parity split) or single windows.
+    const bool split = topk && excl >= 2;
+    const int64_t bucket_len = split ? 2 * excl : 1;
+    const int64_t nb = topk ? (L + bucket_len - 1) / bucket_len : 0;
+    const std::vector<Round> rounds = plan_rounds(L, topk);
+    int64_t max_round = 0;
+    for (const Round& r : rounds) max_round = std::max(max_round, r.count);
     const size_t bytes = carve_size({own_stats ? sizeof(float) * L : 0,
                                      own_stats ? sizeof(float) * L : 0,
-                                     sizeof(int32_t) * L,
+                                     sizeof(int32_t) * max_round,
                                      topk ? sizeof(unsigned long long) * L : 0,
-                                     sizeof(float) * sp.count});
+                                     sizeof(float) * sp.count,
+                                     topk ? sizeof(uint32_t) * nb : 0,
+                                     topk ? sizeof(uint32_t) * tighten_ws_words() : 0});
     void* ws = nullptr;
     rc = ws_reserve(e, bytes, &ws);
     if (rc != MDTW_OK) return rc;
     Carve cv{static_cast<char*>(ws)};
     float* avg = own_stats ? cv.take<float>(L) : nullptr;
     float* sd = own_stats ? cv.take<float>(L) : nullptr;
-    int32_t* cand = cv.take<int32_t>(L);
+    int32_t* cand = cv.take<int32_t>(max_round);
     unsigned long long* res = topk ? cv.take<unsigned long long>(L) : nullptr;
     float* seeds = cv.take<float>(sp.count);
+    uint32_t* bmin = topk ? cv.take<uint32_t>(nb) : nullptr;
+    uint32_t* tscratch = topk ? cv.take<uint32_t>(tighten_ws_words()) : nullptr;
     if (!own_stats) {
         avg = const_cast<float*>(d_avg);
         sd = const_cast<float*>(d_std);
@@ -134,7 +170,10 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
 
     hipStream_t st = e->stream;
     rc = hip_check(hipEventRecord(e->ev0, st));
-    if (rc == MDTW_OK) rc = launch_ctrl_reset(st, e->d_ctrl, kBestNone);
+    if (rc == MDTW_OK) rc = launch_ctrl_reset(st, e->d_ctrl, kBestNone, bmin, bucket_len);
+    if (rc == MDTW_OK && topk) rc = launch_fill_u32(st, bmin, nb, 0x7f800000u);
+    if (rc == MDTW_OK && topk)
+        rc = launch_fill_u32(st, tscratch, static_cast<int64_t>(tighten_ws_words()), 0u);
     if (rc == MDTW_OK && own_stats) {
         rc = stats_mode == MDTW_STATS_REFCOMPAT
                  ? launch_stats_refcompat(e, d_series, n, m, avg, sd)
@@ -163,33 +202,41 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     if (rc == MDTW_OK && topk) rc = launch_seed_threshold(st, seeds, sp.count, k, e->d_ctrl);
     if (rc != MDTW_OK) return rc;
 
-    if (algo == MDTW_ALGO_ED) {
-        // ED is cheap per window and its own early abandon is the filter.
-        BandArgs fa = a;
-        fa.count = L;
-        fa.mode = topk ? kModeCollect : kModeBest;
-        rc = run_algo(e, algo, fa, wc, metric);
-    } else if (m >= 11) {
-        // 3. cascade, 4. DP over survivors.
-        rc = launch_cascade(st, d_series, avg, sd, e->d_q, e->d_lo, e->d_up, m, L, metric,
-                            !topk, e->d_ctrl, cand);
-        if (rc == MDTW_OK) {
+    // 3-4. rounds over growing window ranges: the bound left by earlier
+    // rounds (k = 1: the best distance so far; k > 1: the bucket bound)
+    // prunes the later, larger ones.
+    for (const Round& r : rounds) {
+        if (algo == MDTW_ALGO_ED) {
+            // ED is cheap per window and its own early abandon is the filter.
+            BandArgs fa = a;
+            fa.base = r.base;
+            fa.count = r.count;
+            fa.mode = topk ? kModeCollect : kModeBest;
+            rc = run_algo(e, algo, fa, wc, metric);
+        } else if (m >= 11) {
+            rc = launch_round_reset(st, e->d_ctrl);
+            if (rc == MDTW_OK)
+                rc = launch_cascade(st, d_series, avg, sd, e->d_q, e->d_lo, e->d_up, m, r.base,
+                                    r.count, metric, !topk, e->d_ctrl, cand);
+            if (rc == MDTW_OK) {
+                BandArgs da = a;
+                da.idx = cand;
+                da.idx_count_from_ctrl = 1;
+                da.persistent = 1;
+                da.mode = topk ? kModeCollect : kModeBest;
+                rc = run_algo(e, algo, da, wc, metric);
+            }
+        } else {
+            // LB_Kim needs M >= 11 (bounds.cuh:418): evaluate every window.
             BandArgs da = a;
-            da.idx = cand;
-            da.idx_count_from_ctrl = 1;
-            da.persistent = 1;
+            da.base = r.base;
+            da.count = r.count;
             da.mode = topk ? kModeCollect : kModeBest;
             rc = run_algo(e, algo, da, wc, metric);
         }
-    } else {
-        // LB_Kim needs M >= 11 (bounds.cuh:418): evaluate every window.
-        BandArgs da = a;
-        da.count = L;
-        da.persistent = algo == MDTW_ALGO_ED ? 0 : 1;
-        da.mode = topk ? kModeCollect : kModeBest;
-        rc = run_algo(e, algo, da, wc, metric);
+        if (rc == MDTW_OK && topk) rc = launch_tighten(st, bmin, nb, k, split, tscratch, e->d_ctrl);
+        if (rc != MDTW_OK) return rc;
     }
-    if (rc != MDTW_OK) return rc;
     rc = hip_check(hipEventRecord(e->ev1, st));
     if (rc == MDTW_OK)
         rc = hip_check(hipMemcpyAsync(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));

@@ -8,8 +8,11 @@
 namespace mdtw {
 namespace {
 
-__global__ void ctrl_reset_kernel(Ctrl* ctrl, unsigned long long best_init) {
+__global__ void ctrl_reset_kernel(Ctrl* ctrl, unsigned long long best_init, uint32_t* bmin,
+                                  long long bucket_len) {
     if (threadIdx.x == 0) {
+        ctrl->bmin = bmin;
+        ctrl->bucket_len = bucket_len;
         ctrl->best = best_init;
         ctrl->count = 0;
         ctrl->work = 0;
@@ -152,10 +155,155 @@ inline int check_launch() {
     return MDTW_OK;
 }
 
+__global__ void round_reset_kernel(Ctrl* ctrl) {
+    if (threadIdx.x == 0) {
+        ctrl->count = 0;
+        ctrl->work = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += stride)
+        p[i] = v;
+}
+
+// ---- top-k bound: two-pass radix selection over the bucket minima ----------
+// Pass 0 histograms bits 31..20 of every bucket minimum (non-negative float
+// bits order like the values), per parity class; the k-th smallest lies in
+// the first bin where the running count reaches k.  Pass 1 histograms bits
+// 19..8 of the minima inside that bin.  The bound is the largest value with
+// the 24 selected bits, so it is >= the k-th smallest (within 2^-15 of it).
+// Scratch layout (uint32): hist[2][4096], then state: prefix[2], krem[2].
+constexpr int kSelBins = 4096;
+constexpr int kSelState = 2 * kSelBins;
+
+__global__ __launch_bounds__(256) void tighten_hist_kernel(const uint32_t* __restrict__ bmin,
+                                                           int64_t nb, int split, int pass,
+                                                           uint32_t* __restrict__ ws) {
+    __shared__ uint32_t h[2 * kSelBins];
+    for (int i = threadIdx.x; i < 2 * kSelBins; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint32_t p0 = ws[kSelState + 0], p1 = ws[kSelState + 1];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < nb;
+         b += stride) {
+        const uint32_t v = bmin[b];
+        const int par = split ? static_cast<int>(b & 1) : 0;
+        if (pass == 0) {
+            atomicAdd(&h[par * kSelBins + (v >> 20)], 1u);
+        } else if ((v >> 20) == (par ? p1 : p0)) {
+            atomicAdd(&h[par * kSelBins + ((v >> 8) & 0xfffu)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * kSelBins; i += blockDim.x)
+        if (h[i]) atomicAdd(&ws[i], h[i]);
+}
+
+// One block: find the bin of the remaining rank per parity, clear the
+// histogram for the next pass; after pass 1 lower ctrl->thr.
+__global__ __launch_bounds__(256) void tighten_find_kernel(uint32_t* __restrict__ ws, int32_t k,
+                                                           int split, int pass,
+                                                           Ctrl* __restrict__ ctrl) {
+    __shared__ uint32_t part[256];
+    __shared__ uint32_t found[2];
+    const int t = threadIdx.x;
+    constexpr int per = kSelBins / 256;
+    for (int par = 0; par < (split ? 2 : 1); ++par) {
+        uint32_t* h = ws + par * kSelBins;
+        const uint32_t krem = pass == 0 ? static_cast<uint32_t>(k) : ws[kSelState + 2 + par];
+        uint32_t sum = 0;
+        for (int i = 0; i < per; ++i) sum += h[t * per + i];
+        part[t] = sum;
+        __syncthreads();
+        if (t == 0) {
+            // Running count over blocks of bins, then inside the block.
+            uint32_t run = 0;
+            int blk = -1;
+            for (int i = 0; i < 256; ++i) {
+                if (run + part[i] >= krem) {
+                    blk = i;
+                    break;
+                }
+                run += part[i];
+            }
+            uint32_t bin = 0xffffffffu;
+            if (blk >= 0) {
+                for (int i = 0; i < per; ++i) {
+                    const uint32_t c = h[blk * per + i];
+                    if (run + c >= krem) {
+                        bin = static_cast<uint32_t>(blk * per + i);
+                        break;
+                    }
+                    run += c;
+                }
+            }
+            found[par] = bin;
+            ws[kSelState + 2 + par] = krem - run;  // rank inside the bin
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < 2 * kSelBins; i += blockDim.x) ws[i] = 0;
+    if (t != 0) return;
+    uint32_t bound[2] = {0x7f800000u, 0x7f800000u};
+    for (int par = 0; par < (split ? 2 : 1); ++par) {
+        if (found[par] == 0xffffffffu) {  // fewer than k known: no bound
+            ws[kSelState + par] = 0xffffffffu;
+            continue;
+        }
+        if (pass == 0) {
+            ws[kSelState + par] = found[par];
+        } else {
+            const uint32_t hi = (ws[kSelState + par] << 20) | (found[par] << 8) | 0xffu;
+            bound[par] = hi >= 0x7f800000u ? 0x7f800000u : hi;
+        }
+    }
+    if (pass == 1) {
+        uint32_t b = bound[0];
+        if (split && bound[1] < b) b = bound[1];
+        const float tb = __uint_as_float(b);
+        if (tb < ctrl->thr) ctrl->thr = tb;
+    }
+}
+
 }  // namespace
 
-int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init) {
-    hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, st, ctrl, best_init);
+int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init, uint32_t* bmin,
+                      int64_t bucket_len) {
+    hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, st, ctrl, best_init, bmin,
+                       static_cast<long long>(bucket_len));
+    return check_launch();
+}
+
+int launch_round_reset(hipStream_t st, Ctrl* ctrl) {
+    hipLaunchKernelGGL(round_reset_kernel, dim3(1), dim3(64), 0, st, ctrl);
+    return check_launch();
+}
+
+int launch_fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v) {
+    if (n <= 0) return MDTW_OK;
+    const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, st, p, n,
+                       v);
+    return check_launch();
+}
+
+size_t tighten_ws_words() { return kSelState + 4; }
+
+// scratch must hold zeros in its histogram part on entry (launch_fill_u32
+// once); every pass leaves it cleared again.
+int launch_tighten(hipStream_t st, const uint32_t* bmin, int64_t nb, int32_t k, bool split,
+                   uint32_t* scratch, Ctrl* ctrl) {
+    if (nb <= 0) return MDTW_OK;
+    const int64_t blocks = std::min<int64_t>((nb + 255) / 256, 1024);
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(tighten_hist_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                           st, bmin, nb, split ? 1 : 0, pass, scratch);
+        hipLaunchKernelGGL(tighten_find_kernel, dim3(1), dim3(256), 0, st, scratch, k,
+                           split ? 1 : 0, pass, ctrl);
+    }
     return check_launch();
 }
 
