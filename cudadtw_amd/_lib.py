@@ -83,6 +83,12 @@ SIGNATURES = {
                                      _P]),
     "mdtw_search": (C.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64,
                               C.POINTER(_I64), _FP, C.POINTER(SearchInfo)]),
+    "mdtw_batch_create": (C.c_int, [_P, _FP, C.POINTER(_I32), C.POINTER(_I32), _I32, _I32,
+                                    C.POINTER(_P)]),
+    "mdtw_batch_destroy": (C.c_int, [_P]),
+    "mdtw_batch_tune": (C.c_int, [_P, _P, _I64, _I32, _I32, _I64, C.POINTER(_I32)]),
+    "mdtw_batch_search": (C.c_int, [_P, _P, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64),
+                                    _FP, C.POINTER(SearchInfo), _FP]),
     "mdtw_threshold": (C.c_int, [_P, _P, _P, _I64, _F, C.POINTER(_I64)]),
     "mdtw_argmin": (C.c_int, [_P, _P, _I64, C.POINTER(_I64), _FP]),
 }

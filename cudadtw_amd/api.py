@@ -25,7 +25,8 @@ from . import _lib
 from ._lib import (ALGO_CDTW, ALGO_ED, ALGO_LDTW, METRIC_L1, METRIC_L2, STATS_FP64,
                    STATS_REFCOMPAT, check)
 
-__all__ = ["Engine", "SearchResult", "znormalize", "envelope", "band_width",
+__all__ = ["Engine", "SearchResult", "Batch", "BatchResult", "znormalize", "envelope",
+           "band_width",
            "ALGO_CDTW", "ALGO_ED", "ALGO_LDTW", "METRIC_L1", "METRIC_L2",
            "STATS_FP64", "STATS_REFCOMPAT"]
 
@@ -285,6 +286,12 @@ class Engine:
         r = self.search(subject, 0, ALGO_ED, METRIC_L2, 1, 0, avg, sd)
         return r.location, r.distance
 
+    # -- batched search ----------------------------------------------------
+    def batch(self, zqueries, bands, pipelines: int = 0) -> "Batch":
+        """Queries of mixed length and band searched together against one
+        stream (mdtw_batch_*; no reference counterpart)."""
+        return Batch(self, zqueries, bands, pipelines)
+
     # -- primitives --------------------------------------------------------
     def threshold(self, values, indices, thr: float) -> int:
         """cub_util.cuh:154-172 threshold: stable in-place compaction of the
@@ -308,3 +315,72 @@ class Engine:
 
     def synchronize(self):
         check(self.lib.mdtw_synchronize(self.h), "mdtw_synchronize")
+
+
+@dataclass
+class BatchResult:
+    positions: np.ndarray   # int64[nq, k], -1 where fewer than k matches exist
+    distances: np.ndarray   # float32[nq, k]
+    infos: list             # per-query search info (dict)
+    elapsed_ms: float       # device time of the whole batch (one graph launch)
+
+
+class Batch:
+    """A set of z-normalised queries of mixed length and band, searched
+    together: every query gets mdtw_search's answer, the searches run on
+    concurrent pipelines replayed from one captured hipGraph."""
+
+    def __init__(self, engine: Engine, zqueries, bands, pipelines: int = 0):
+        self.engine = engine
+        qs = [np.ascontiguousarray(q, dtype=np.float32) for q in zqueries]
+        self.lengths = np.array([q.size for q in qs], dtype=np.int32)
+        self.bands = np.ascontiguousarray(bands, dtype=np.int32)
+        if self.bands.size != len(qs):
+            raise ValueError("one band per query")
+        packed = np.concatenate(qs) if qs else np.zeros(0, np.float32)
+        h = C.c_void_p()
+        engine._stream()
+        check(engine.lib.mdtw_batch_create(
+            engine.h, _fp(packed), self.lengths.ctypes.data_as(C.POINTER(C.c_int32)),
+            self.bands.ctypes.data_as(C.POINTER(C.c_int32)), len(qs), int(pipelines),
+            C.byref(h)), "mdtw_batch_create")
+        self.h = h
+        self.nq = len(qs)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.engine.lib.mdtw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tune(self, series, k: int = 1, metric: int = METRIC_L2, cell_budget: int = 0) -> int:
+        """Pick launch shapes per (length class, band kind) bucket by timing."""
+        eng = self.engine
+        nb = C.c_int32()
+        eng._stream()
+        check(eng.lib.mdtw_batch_tune(self.h, eng._f32(series, "series"), series.numel(),
+                                      int(metric), int(k), int(cell_budget), C.byref(nb)),
+              "mdtw_batch_tune")
+        return nb.value
+
+    def search(self, series, k: int = 1, excl=None, metric: int = METRIC_L2) -> BatchResult:
+        eng = self.engine
+        pos = np.full((self.nq, k), -1, dtype=np.int64)
+        dist = np.full((self.nq, k), np.inf, dtype=np.float32)
+        infos = (_lib.SearchInfo * self.nq)()
+        ms = C.c_float()
+        ex = None
+        if excl is not None:
+            ex = np.ascontiguousarray(np.broadcast_to(np.asarray(excl, np.int64), (self.nq,)))
+        eng._stream()
+        check(eng.lib.mdtw_batch_search(
+            self.h, eng._f32(series, "series"), series.numel(), int(metric), int(k),
+            ex.ctypes.data_as(C.POINTER(C.c_int64)) if ex is not None else None,
+            pos.ctypes.data_as(C.POINTER(C.c_int64)), _fp(dist), infos, C.byref(ms)),
+            "mdtw_batch_search")
+        return BatchResult(pos, dist, [i.as_dict() for i in infos], ms.value)

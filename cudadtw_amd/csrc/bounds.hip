@@ -347,11 +347,13 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg, const floa
 
 int launch_cascade(hipStream_t st, const float* s, const float* avg, const float* sd,
                    const float* q, const float* lo, const float* up, int32_t m, int64_t base,
-                   int64_t L, int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand) {
+                   int64_t L, int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
+                   int64_t max_blocks) {
     if (L <= 0) return MDTW_OK;
     // Enough waves to cover the chip several times over; each wave strides.
     const int64_t waves_needed = (L + kWave - 1) / kWave;
-    const int64_t nb64 = std::min<int64_t>((waves_needed + 3) / 4, 256 * 64);
+    const int64_t nb64 = std::max<int64_t>(
+        1, std::min<int64_t>((waves_needed + kCascadeWaves - 1) / kCascadeWaves, max_blocks));
     const unsigned nb = static_cast<unsigned>(nb64);
     if (metric)
         hipLaunchKernelGGL(cascade_kernel<true>, dim3(nb), dim3(256), 0, st, s, avg, sd, q, lo,

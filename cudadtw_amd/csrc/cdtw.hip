@@ -246,14 +246,14 @@ BandLaunchFn band_lookup(int w, bool l2) {
 // Band kernels need the padded query (qx); the wide kernel the plain query.
 // The caller passes both through BandArgs::qx / q_plain.
 int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain, int32_t w,
-              int32_t metric, bool force_wide) {
+              int32_t metric, bool force_wide, const LaunchShape& shape) {
     const bool l2 = metric != 0;
     const int64_t count_hint = a.idx_count_from_ctrl ? -1 : a.count;
     if (!a.idx_count_from_ctrl && a.count <= 0) return MDTW_OK;
     BandLaunchFn fn = force_wide || a.qx == nullptr ? nullptr : band_lookup(w, l2);
     if (fn) {
         const int64_t per_block = static_cast<int64_t>(kBandWaves) * kWave;
-        int64_t grid = static_cast<int64_t>(cus) * 8;
+        int64_t grid = static_cast<int64_t>(cus) * shape.dp_blocks_per_cu;
         if (!a.persistent) grid = std::min(grid, (count_hint + per_block - 1) / per_block);
         return fn(st, static_cast<unsigned>(grid), a);
     }
@@ -262,7 +262,7 @@ int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain, 
     b.qx = q_plain;
     unsigned grid;
     if (a.persistent)
-        grid = static_cast<unsigned>(cus * 32);
+        grid = static_cast<unsigned>(cus * shape.wide_waves_per_cu);
     else
         grid = static_cast<unsigned>(count_hint);
     return wide_dispatch(st, grid, b, w, l2);

@@ -163,6 +163,60 @@ struct BandScalars {
     int32_t qx_len;  // band kernel: length of the padded query buffer
 };
 
+// Launch shapes of the search kernels; the batch autotuner picks them per
+// query-length / band bucket, plain searches use the defaults.
+struct LaunchShape {
+    int32_t dp_blocks_per_cu = 8;         // persistent register-band DP (4 waves each)
+    int32_t wide_waves_per_cu = 32;       // persistent anti-diagonal DP (1 wave each)
+    int32_t cascade_blocks_per_cu = 64;   // cap of the cascade grid (4 waves each)
+};
+
+// Everything one search needs on the device (enqueue_search).
+struct SearchPlan {
+    hipStream_t st = nullptr;
+    int cus = 0;
+    const float* s = nullptr;
+    int64_t n = 0;
+    const float* d_avg = nullptr;  // given window statistics, or NULL: computed
+    const float* d_std = nullptr;
+    int32_t stats_mode = MDTW_STATS_FP64;
+    mdtw_engine* e = nullptr;      // REFCOMPAT statistics only
+    const float* q = nullptr;      // z-normalised query, M floats
+    const float* lo = nullptr;     // its envelope for band w
+    const float* up = nullptr;
+    const float* qx = nullptr;     // padded query of the register-band kernel
+    int32_t qx_len = 0;
+    int32_t m = 0;
+    int32_t w = 0;                 // band, <= m - 1
+    int32_t algo = MDTW_ALGO_CDTW;
+    int32_t metric = MDTW_METRIC_L2;
+    int32_t k = 1;
+    int64_t excl = 0;
+    Ctrl* ctrl = nullptr;
+    char* ws = nullptr;            // search_ws_bytes(...) of scratch
+    LaunchShape shape;
+};
+struct SearchBufs {
+    float* avg = nullptr;
+    float* sd = nullptr;
+    unsigned long long* res = nullptr;  // top-k keys (ctrl->nres of them)
+    int64_t seeded = 0;
+};
+size_t search_ws_bytes(int64_t L, int32_t k, int64_t excl, bool own_stats);
+int64_t search_seed_count(int64_t L, int32_t k, int64_t excl);
+// Enqueues the whole search on p.st (no host synchronisation, no allocation;
+// safe to capture in a graph when stats_mode is FP64).
+int enqueue_search(const SearchPlan& p, SearchBufs* bufs);
+// Greedy top-k with exclusion over the kept keys on the device (one
+// workgroup): out_pos/out_dist get k entries (position -1 / +inf past the
+// matches found); distances are sqrt'ed for L2.  k = 1 decodes ctrl->best.
+// Sets *overflow when more than kSelectMaxKeys keys are at or below the
+// final bound (the caller then selects on the host).
+constexpr int32_t kSelectMaxKeys = 8192;
+int launch_topk_select(hipStream_t st, const Ctrl* ctrl, const unsigned long long* res,
+                       int32_t k, int64_t excl, int32_t metric, int64_t* out_pos,
+                       float* out_dist, int32_t* overflow);
+
 // ---- kernel launchers (each in its own translation unit) -----------------
 // stats.hip
 int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,
@@ -182,11 +236,13 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg,
 int launch_cascade(hipStream_t st, const float* s, const float* avg,
                    const float* sd, const float* q, const float* lo,
                    const float* up, int32_t m, int64_t base, int64_t count,
-                   int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand);
+                   int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
+                   int64_t max_blocks = 256 * 64);
 // cdtw.hip: band kernel (qx = padded query) or, for wide bands, the
 // anti-diagonal kernel (q_plain).
 int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain,
-              int32_t w, int32_t metric, bool force_wide);
+              int32_t w, int32_t metric, bool force_wide,
+              const LaunchShape& shape = LaunchShape());
 // ldtw.hip (a.qx = plain query)
 int launch_ldtw(hipStream_t st, int cus, const BandArgs& a, int32_t w, int32_t metric);
 // ed.hip (a.qx = plain query; no persistent mode)

@@ -83,16 +83,179 @@ int hip_check(hipError_t err) {
     return MDTW_OK;
 }
 
+struct TopkLayout {
+    bool split;
+    int64_t bucket_len;
+    int64_t nb;
+};
+
+TopkLayout topk_layout(int64_t L, int32_t k, int64_t excl) {
+    // Top-k bound buckets: 2*excl windows (parity split) or single windows.
+    TopkLayout t;
+    t.split = k > 1 && excl >= 2;
+    t.bucket_len = t.split ? 2 * excl : 1;
+    t.nb = k > 1 ? (L + t.bucket_len - 1) / t.bucket_len : 0;
+    return t;
+}
+
+// Greedy top-k with the exclusion zone over the kept (distance, index)
+// keys: ascending keys, a key is taken unless within |i - j| < excl of an
+// earlier pick.  Returns the number of picks.
+int32_t host_topk_select(std::vector<unsigned long long>& keys, int32_t k, int64_t excl,
+                         int32_t metric, int64_t* h_pos, float* h_dist) {
+    std::sort(keys.begin(), keys.end());
+    int32_t found = 0;
+    for (unsigned long long key : keys) {
+        if (found == k) break;
+        const int64_t pos = static_cast<int64_t>(key & 0xffffffffull);
+        bool ok = true;
+        for (int32_t p = 0; p < found; ++p) {
+            int64_t dd = pos - h_pos[p];
+            if (dd < 0) dd = -dd;
+            if (dd < excl) {
+                ok = false;
+                break;
+            }
+        }
+        if (!ok) continue;
+        const uint32_t bits = static_cast<uint32_t>(key >> 32);
+        float d;
+        std::memcpy(&d, &bits, sizeof(d));
+        h_pos[found] = pos;
+        h_dist[found] = metric == MDTW_METRIC_L2 ? std::sqrt(d) : d;
+        found++;
+    }
+    return found;
+}
+
 // One DP-style launch for the selected algorithm.
-int run_algo(mdtw_engine* e, int32_t algo, const BandArgs& a, int32_t w, int32_t metric) {
-    if (algo == MDTW_ALGO_CDTW) return launch_dp(e->stream, e->cus, a, e->d_q, w, metric, false);
+int run_algo(const SearchPlan& p, const BandArgs& a) {
+    if (p.algo == MDTW_ALGO_CDTW)
+        return launch_dp(p.st, p.cus, a, p.q, p.w, p.metric, false, p.shape);
     BandArgs b = a;
-    b.qx = e->d_q;
-    if (algo == MDTW_ALGO_LDTW) return launch_ldtw(e->stream, e->cus, b, w, metric);
-    return launch_ed(e->stream, e->cus, b);
+    b.qx = p.q;
+    if (p.algo == MDTW_ALGO_LDTW) return launch_ldtw(p.st, p.cus, b, p.w, p.metric);
+    return launch_ed(p.st, p.cus, b);
 }
 
 }  // namespace
+
+namespace mdtw {
+
+int64_t search_seed_count(int64_t L, int32_t k, int64_t excl) {
+    return plan_seeds(L, k, excl).count;
+}
+
+size_t search_ws_bytes(int64_t L, int32_t k, int64_t excl, bool own_stats) {
+    const SeedPlan sp = plan_seeds(L, k, excl);
+    const bool topk = k > 1;
+    const TopkLayout tl = topk_layout(L, k, excl);
+    int64_t max_round = 0;
+    for (const Round& r : plan_rounds(L, topk)) max_round = std::max(max_round, r.count);
+    return carve_size({own_stats ? sizeof(float) * L : 0, own_stats ? sizeof(float) * L : 0,
+                       sizeof(int32_t) * max_round,
+                       topk ? sizeof(unsigned long long) * L : 0, sizeof(float) * sp.count,
+                       topk ? sizeof(uint32_t) * tl.nb : 0,
+                       topk ? sizeof(uint32_t) * tighten_ws_words() : 0});
+}
+
+int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
+    const int32_t m = p.m;
+    const int64_t L = p.n - m + 1;
+    const int32_t k = p.k;
+    const bool topk = k > 1;
+    const bool own_stats = p.d_avg == nullptr;
+    const SeedPlan sp = plan_seeds(L, k, p.excl);
+    const TopkLayout tl = topk_layout(L, k, p.excl);
+    const std::vector<Round> rounds = plan_rounds(L, topk);
+    int64_t max_round = 0;
+    for (const Round& r : rounds) max_round = std::max(max_round, r.count);
+    Carve cv{p.ws};
+    float* avg = own_stats ? cv.take<float>(L) : const_cast<float*>(p.d_avg);
+    float* sd = own_stats ? cv.take<float>(L) : const_cast<float*>(p.d_std);
+    int32_t* cand = cv.take<int32_t>(max_round);
+    unsigned long long* res = topk ? cv.take<unsigned long long>(L) : nullptr;
+    float* seeds = cv.take<float>(sp.count);
+    uint32_t* bmin = topk ? cv.take<uint32_t>(tl.nb) : nullptr;
+    uint32_t* tscratch = topk ? cv.take<uint32_t>(tighten_ws_words()) : nullptr;
+    bufs->avg = avg;
+    bufs->sd = sd;
+    bufs->res = res;
+    bufs->seeded = sp.count;
+
+    hipStream_t st = p.st;
+    int rc = launch_ctrl_reset(st, p.ctrl, kBestNone, bmin, tl.bucket_len);
+    if (rc == MDTW_OK && topk) rc = launch_fill_u32(st, bmin, tl.nb, 0x7f800000u);
+    if (rc == MDTW_OK && topk)
+        rc = launch_fill_u32(st, tscratch, static_cast<int64_t>(tighten_ws_words()), 0u);
+    if (rc == MDTW_OK && own_stats) {
+        rc = p.stats_mode == MDTW_STATS_REFCOMPAT
+                 ? launch_stats_refcompat(p.e, p.s, p.n, m, avg, sd)
+                 : launch_stats_fp64(st, p.s, p.n, m, avg, sd);
+    }
+    if (rc != MDTW_OK) return rc;
+
+    BandArgs a;
+    a.s = p.s;
+    a.avg = avg;
+    a.sd = sd;
+    a.qx = p.qx;
+    a.qx_len = p.qx_len;
+    a.m = m;
+    a.ctrl = p.ctrl;
+    a.res = res;
+
+    // 2. seed
+    BandArgs sa = a;
+    sa.count = sp.count;
+    sa.base = 0;
+    sa.stride = sp.stride;
+    sa.mode = topk ? kModeSeed : kModeBest;
+    sa.out = seeds;
+    rc = run_algo(p, sa);
+    if (rc == MDTW_OK && topk) rc = launch_seed_threshold(st, seeds, sp.count, k, p.ctrl);
+    if (rc != MDTW_OK) return rc;
+
+    // 3. rounds: the bound left by earlier rounds (k = 1: the best distance
+    // so far; k > 1: the bucket bound) prunes the later ones.
+    const int64_t cascade_cap = static_cast<int64_t>(p.cus) * p.shape.cascade_blocks_per_cu;
+    for (const Round& r : rounds) {
+        if (p.algo == MDTW_ALGO_ED) {
+            // ED is cheap per window and its own early abandon is the filter.
+            BandArgs fa = a;
+            fa.base = r.base;
+            fa.count = r.count;
+            fa.mode = topk ? kModeCollect : kModeBest;
+            rc = run_algo(p, fa);
+        } else if (m >= 11) {
+            rc = launch_round_reset(st, p.ctrl);
+            if (rc == MDTW_OK)
+                rc = launch_cascade(st, p.s, avg, sd, p.q, p.lo, p.up, m, r.base, r.count,
+                                    p.metric, !topk, p.ctrl, cand, cascade_cap);
+            if (rc == MDTW_OK) {
+                BandArgs da = a;
+                da.idx = cand;
+                da.idx_count_from_ctrl = 1;
+                da.persistent = 1;
+                da.mode = topk ? kModeCollect : kModeBest;
+                rc = run_algo(p, da);
+            }
+        } else {
+            // LB_Kim needs M >= 11 (bounds.cuh:418): evaluate every window.
+            BandArgs da = a;
+            da.base = r.base;
+            da.count = r.count;
+            da.mode = topk ? kModeCollect : kModeBest;
+            rc = run_algo(p, da);
+        }
+        if (rc == MDTW_OK && topk)
+            rc = launch_tighten(st, bmin, tl.nb, k, tl.split, tscratch, p.ctrl);
+        if (rc != MDTW_OK) return rc;
+    }
+    return MDTW_OK;
+}
+
+}  // namespace mdtw
 
 extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, const float* d_avg,
                            const float* d_std, int32_t w, int32_t algo, int32_t metric,
@@ -117,15 +280,7 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
         h_pos[i] = -1;
         h_dist[i] = INFINITY;
     }
-    int dev_prev = -1;
-    if (hipGetDevice(&dev_prev) == hipSuccess && dev_prev != e->device) (void)hipSetDevice(e->device);
-    else dev_prev = -1;
-    struct Restore {
-        int d;
-        ~Restore() {
-            if (d >= 0) (void)hipSetDevice(d);
-        }
-    } restore{dev_prev};
+    DeviceGuard guard(e->device);
 
     // Query-side state first (host work and small uploads, outside timing).
     int rc = MDTW_OK;
@@ -134,115 +289,43 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
         if (rc == MDTW_OK && algo == MDTW_ALGO_CDTW) rc = ensure_qx(e, wc);
         if (rc != MDTW_OK) return rc;
     }
-
-    const SeedPlan sp = plan_seeds(L, k, excl);
-    const bool own_stats = d_avg == nullptr;
-    const bool topk = k > 1;
-    // Top-k bound buckets: 2*excl windows (parity split) or single windows.
-    const bool split = topk && excl >= 2;
-    const int64_t bucket_len = split ? 2 * excl : 1;
-    const int64_t nb = topk ? (L + bucket_len - 1) / bucket_len : 0;
-    const std::vector<Round> rounds = plan_rounds(L, topk);
-    int64_t max_round = 0;
-    for (const Round& r : rounds) max_round = std::max(max_round, r.count);
-    const size_t bytes = carve_size({own_stats ? sizeof(float) * L : 0,
-                                     own_stats ? sizeof(float) * L : 0,
-                                     sizeof(int32_t) * max_round,
-                                     topk ? sizeof(unsigned long long) * L : 0,
-                                     sizeof(float) * sp.count,
-                                     topk ? sizeof(uint32_t) * nb : 0,
-                                     topk ? sizeof(uint32_t) * tighten_ws_words() : 0});
     void* ws = nullptr;
-    rc = ws_reserve(e, bytes, &ws);
+    rc = ws_reserve(e, search_ws_bytes(L, k, excl, d_avg == nullptr), &ws);
     if (rc != MDTW_OK) return rc;
-    Carve cv{static_cast<char*>(ws)};
-    float* avg = own_stats ? cv.take<float>(L) : nullptr;
-    float* sd = own_stats ? cv.take<float>(L) : nullptr;
-    int32_t* cand = cv.take<int32_t>(max_round);
-    unsigned long long* res = topk ? cv.take<unsigned long long>(L) : nullptr;
-    float* seeds = cv.take<float>(sp.count);
-    uint32_t* bmin = topk ? cv.take<uint32_t>(nb) : nullptr;
-    uint32_t* tscratch = topk ? cv.take<uint32_t>(tighten_ws_words()) : nullptr;
-    if (!own_stats) {
-        avg = const_cast<float*>(d_avg);
-        sd = const_cast<float*>(d_std);
-    }
 
+    SearchPlan p;
+    p.st = e->stream;
+    p.cus = e->cus;
+    p.s = d_series;
+    p.n = n;
+    p.d_avg = d_avg;
+    p.d_std = d_std;
+    p.stats_mode = stats_mode;
+    p.e = e;
+    p.q = e->d_q;
+    p.lo = e->d_lo;
+    p.up = e->d_up;
+    p.qx = e->d_qx;
+    p.qx_len = e->qx_len;
+    p.m = m;
+    p.w = wc;
+    p.algo = algo;
+    p.metric = metric;
+    p.k = k;
+    p.excl = excl;
+    p.ctrl = e->d_ctrl;
+    p.ws = static_cast<char*>(ws);
+    SearchBufs bufs;
     hipStream_t st = e->stream;
     rc = hip_check(hipEventRecord(e->ev0, st));
-    if (rc == MDTW_OK) rc = launch_ctrl_reset(st, e->d_ctrl, kBestNone, bmin, bucket_len);
-    if (rc == MDTW_OK && topk) rc = launch_fill_u32(st, bmin, nb, 0x7f800000u);
-    if (rc == MDTW_OK && topk)
-        rc = launch_fill_u32(st, tscratch, static_cast<int64_t>(tighten_ws_words()), 0u);
-    if (rc == MDTW_OK && own_stats) {
-        rc = stats_mode == MDTW_STATS_REFCOMPAT
-                 ? launch_stats_refcompat(e, d_series, n, m, avg, sd)
-                 : launch_stats_fp64(st, d_series, n, m, avg, sd);
-    }
-    if (rc != MDTW_OK) return rc;
-
-    BandArgs a;
-    a.s = d_series;
-    a.avg = avg;
-    a.sd = sd;
-    a.qx = e->d_qx;
-    a.qx_len = e->qx_len;
-    a.m = m;
-    a.ctrl = e->d_ctrl;
-    a.res = res;
-
-    // 2. seed
-    BandArgs sa = a;
-    sa.count = sp.count;
-    sa.base = 0;
-    sa.stride = sp.stride;
-    sa.mode = topk ? kModeSeed : kModeBest;
-    sa.out = seeds;
-    rc = run_algo(e, algo, sa, wc, metric);
-    if (rc == MDTW_OK && topk) rc = launch_seed_threshold(st, seeds, sp.count, k, e->d_ctrl);
-    if (rc != MDTW_OK) return rc;
-
-    // 3-4. rounds over growing window ranges: the bound left by earlier
-    // rounds (k = 1: the best distance so far; k > 1: the bucket bound)
-    // prunes the later, larger ones.
-    for (const Round& r : rounds) {
-        if (algo == MDTW_ALGO_ED) {
-            // ED is cheap per window and its own early abandon is the filter.
-            BandArgs fa = a;
-            fa.base = r.base;
-            fa.count = r.count;
-            fa.mode = topk ? kModeCollect : kModeBest;
-            rc = run_algo(e, algo, fa, wc, metric);
-        } else if (m >= 11) {
-            rc = launch_round_reset(st, e->d_ctrl);
-            if (rc == MDTW_OK)
-                rc = launch_cascade(st, d_series, avg, sd, e->d_q, e->d_lo, e->d_up, m, r.base,
-                                    r.count, metric, !topk, e->d_ctrl, cand);
-            if (rc == MDTW_OK) {
-                BandArgs da = a;
-                da.idx = cand;
-                da.idx_count_from_ctrl = 1;
-                da.persistent = 1;
-                da.mode = topk ? kModeCollect : kModeBest;
-                rc = run_algo(e, algo, da, wc, metric);
-            }
-        } else {
-            // LB_Kim needs M >= 11 (bounds.cuh:418): evaluate every window.
-            BandArgs da = a;
-            da.base = r.base;
-            da.count = r.count;
-            da.mode = topk ? kModeCollect : kModeBest;
-            rc = run_algo(e, algo, da, wc, metric);
-        }
-        if (rc == MDTW_OK && topk) rc = launch_tighten(st, bmin, nb, k, split, tscratch, e->d_ctrl);
-        if (rc != MDTW_OK) return rc;
-    }
-    rc = hip_check(hipEventRecord(e->ev1, st));
+    if (rc == MDTW_OK) rc = enqueue_search(p, &bufs);
+    if (rc == MDTW_OK) rc = hip_check(hipEventRecord(e->ev1, st));
     if (rc == MDTW_OK)
         rc = hip_check(hipMemcpyAsync(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     if (rc == MDTW_OK) rc = hip_check(hipStreamSynchronize(st));
     if (rc != MDTW_OK) return rc;
     const Ctrl c = *e->h_ctrl;
+    const bool topk = k > 1;
 
     int32_t found = 0;
     float thr_out;
@@ -262,38 +345,19 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
         thr_out = c.thr;
         std::vector<unsigned long long> keys(static_cast<size_t>(c.nres));
         if (c.nres) {
-            rc = hip_check(hipMemcpyAsync(keys.data(), res, sizeof(unsigned long long) * c.nres,
+            rc = hip_check(hipMemcpyAsync(keys.data(), bufs.res,
+                                          sizeof(unsigned long long) * c.nres,
                                           hipMemcpyDeviceToHost, st));
             if (rc == MDTW_OK) rc = hip_check(hipStreamSynchronize(st));
             if (rc != MDTW_OK) return rc;
         }
-        std::sort(keys.begin(), keys.end());
-        for (unsigned long long key : keys) {
-            if (found == k) break;
-            const int64_t pos = static_cast<int64_t>(key & 0xffffffffull);
-            bool ok = true;
-            for (int32_t p = 0; p < found; ++p) {
-                int64_t dd = pos - h_pos[p];
-                if (dd < 0) dd = -dd;
-                if (dd < excl) {
-                    ok = false;
-                    break;
-                }
-            }
-            if (!ok) continue;
-            const uint32_t bits = static_cast<uint32_t>(key >> 32);
-            float d;
-            std::memcpy(&d, &bits, sizeof(d));
-            h_pos[found] = pos;
-            h_dist[found] = metric == MDTW_METRIC_L2 ? std::sqrt(d) : d;
-            found++;
-        }
+        found = host_topk_select(keys, k, excl, metric, h_pos, h_dist);
     }
     if (info) {
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
         info->windows = L;
-        info->seeded = sp.count;
+        info->seeded = bufs.seeded;
         info->lb_kim_pruned = static_cast<int64_t>(c.kim);
         info->lb_keogh_pruned = static_cast<int64_t>(c.keogh);
         info->dp_evaluated = static_cast<int64_t>(c.dp);

@@ -268,7 +268,110 @@ __global__ __launch_bounds__(256) void tighten_find_kernel(uint32_t* __restrict_
     }
 }
 
+// ---- greedy top-k selection on the device ---------------------------------
+constexpr int kSelThreads = 1024;
+
+__global__ __launch_bounds__(kSelThreads) void topk_select_kernel(
+    const Ctrl* __restrict__ ctrl, const unsigned long long* __restrict__ res, int32_t k,
+    long long excl, int32_t metric, int64_t* __restrict__ out_pos, float* __restrict__ out_dist,
+    int32_t* __restrict__ overflow) {
+    __shared__ unsigned long long keys[kSelectMaxKeys];
+    __shared__ int32_t nk;
+    const int t = threadIdx.x;
+    const bool l2 = metric == MDTW_METRIC_L2;
+    if (k == 1) {
+        if (t == 0) {
+            const unsigned long long b = ctrl->best;
+            if ((b >> 32) < 0x7f800000ull) {
+                const float d = key_dist(b);
+                out_pos[0] = static_cast<int64_t>(b & 0xffffffffull);
+                out_dist[0] = l2 ? sqrtf(d) : d;
+            } else {
+                out_pos[0] = -1;
+                out_dist[0] = kInf;
+            }
+            *overflow = 0;
+        }
+        return;
+    }
+    // Keys above the final bound cannot be among the first k picks: the
+    // bound is >= the k-th pick's distance.
+    const float thr = ctrl->thr;
+    const long long nres = static_cast<long long>(ctrl->nres);
+    if (t == 0) nk = 0;
+    __syncthreads();
+    for (long long i = t; i < nres; i += kSelThreads) {
+        const unsigned long long key = res[i];
+        if (key_dist(key) <= thr) {
+            const int32_t slot = atomicAdd(&nk, 1);
+            if (slot < kSelectMaxKeys) keys[slot] = key;
+        }
+    }
+    __syncthreads();
+    const int32_t n = nk;
+    if (n > kSelectMaxKeys) {
+        for (int32_t i = t; i < k; i += kSelThreads) {
+            out_pos[i] = -1;
+            out_dist[i] = kInf;
+        }
+        if (t == 0) *overflow = 1;
+        return;
+    }
+    int32_t P = 1;
+    while (P < n) P <<= 1;
+    for (int32_t i = n + t; i < P; i += kSelThreads) keys[i] = ~0ull;
+    __syncthreads();
+    // Bitonic sort of keys[0..P), ascending.
+    for (int32_t size = 2; size <= P; size <<= 1) {
+        for (int32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int32_t i = t; i < P / 2; i += kSelThreads) {
+                const int32_t lo = 2 * i - (i & (stride - 1));
+                const int32_t hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const unsigned long long a = keys[lo], b = keys[hi];
+                if ((a > b) == up) {
+                    keys[lo] = b;
+                    keys[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (t != 0) return;
+    int32_t found = 0;
+    for (int32_t i = 0; i < n && found < k; ++i) {
+        const unsigned long long key = keys[i];
+        const int64_t pos = static_cast<int64_t>(key & 0xffffffffull);
+        bool ok = true;
+        for (int32_t p = 0; p < found; ++p) {
+            const int64_t dd = pos > out_pos[p] ? pos - out_pos[p] : out_pos[p] - pos;
+            if (dd < excl) {
+                ok = false;
+                break;
+            }
+        }
+        if (!ok) continue;
+        const float d = key_dist(key);
+        out_pos[found] = pos;
+        out_dist[found] = l2 ? sqrtf(d) : d;
+        ++found;
+    }
+    for (int32_t i = found; i < k; ++i) {
+        out_pos[i] = -1;
+        out_dist[i] = kInf;
+    }
+    *overflow = 0;
+}
+
 }  // namespace
+
+int launch_topk_select(hipStream_t st, const Ctrl* ctrl, const unsigned long long* res,
+                       int32_t k, int64_t excl, int32_t metric, int64_t* out_pos,
+                       float* out_dist, int32_t* overflow) {
+    hipLaunchKernelGGL(topk_select_kernel, dim3(1), dim3(kSelThreads), 0, st, ctrl, res, k,
+                       static_cast<long long>(excl), metric, out_pos, out_dist, overflow);
+    return check_launch();
+}
 
 int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init, uint32_t* bmin,
                       int64_t bucket_len) {

@@ -185,6 +185,40 @@ int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n,
                 int64_t excl, int64_t* h_pos, float* h_dist,
                 mdtw_search_info* info);
 
+/* ---- batched search ------------------------------------------------------- */
+/* No reference counterpart (the reference holds one query per process in
+ * __constant__ memory, cdtw.cuh:9-11).  A batch is a set of queries of mixed
+ * length and band searched together against one stream: every query's search
+ * is the mdtw_search algorithm (results identical to searching the queries
+ * one by one), run on several concurrent pipelines and captured into one
+ * hipGraph that each mdtw_batch_search replays.  The batch belongs to the
+ * engine it was created on; destroy it before the engine. */
+typedef struct mdtw_batch mdtw_batch;
+/* h_zqueries: the nq z-normalised queries (mdtw_znormalize) concatenated;
+ * h_lengths[i] their lengths (1..MDTW_MAX_QUERY), h_bands[i] their absolute
+ * bands (clamped to M - 1; at most 1023).  pipelines: concurrent pipelines
+ * (0 = 4, at most 16).  The host arrays are copied. */
+int mdtw_batch_create(mdtw_engine* e, const float* h_zqueries,
+                      const int32_t* h_lengths, const int32_t* h_bands,
+                      int32_t nq, int32_t pipelines, mdtw_batch** out);
+int mdtw_batch_destroy(mdtw_batch* b);
+/* Picks launch shapes (persistent DP grid, anti-diagonal DP grid, cascade
+ * grid) per bucket of queries (power-of-two length class x register-band or
+ * wide band) by timing the bucket's median query on a stream prefix of
+ * about cell_budget band cells (<= 0: 2^39).  Affects speed only. */
+int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n,
+                    int32_t metric, int32_t k, int64_t cell_budget,
+                    int32_t* h_buckets_tuned);
+/* Best (k = 1) or top-k cDTW matches of every query (mdtw_search semantics,
+ * window statistics computed per query length in FP64).  h_excl: nq
+ * exclusion zones, or NULL for M/2 each.  Outputs: h_pos/h_dist nq*k
+ * (query-major), h_info nq entries (may be NULL; elapsed_ms is the whole
+ * batch), *h_elapsed_ms the device time of the graph (may be NULL). */
+int mdtw_batch_search(mdtw_batch* b, const float* d_series, int64_t n,
+                      int32_t metric, int32_t k, const int64_t* h_excl,
+                      int64_t* h_pos, float* h_dist, mdtw_search_info* h_info,
+                      float* h_elapsed_ms);
+
 /* ---- primitives (cub_util.cuh) ------------------------------------------- */
 /* cub_util.cuh:154-172 threshold: keeps d_indices[i] with d_values[i] <= thr,
  * in order, in place; the new length goes to *h_count. */

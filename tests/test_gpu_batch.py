@@ -1,0 +1,109 @@
+"""Batched search (mdtw_batch_*) against per-query plain searches and the
+oracle.
+
+No reference counterpart (the reference runs one query per process).  The
+bar: every query's batched result -- positions and distances -- is
+identical to its own mdtw_search (the same kernels and bounds, so the same
+bits), which is itself checked against the oracle's full profile
+(test_gpu_search.py); graph replays and tuned launch shapes change nothing.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cudadtw_amd import METRIC_L1, METRIC_L2, band_width, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=torch.float32):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device="cuda", dtype=dtype)
+
+
+def make_queries(stream, specs, seed):
+    """specs: (M, band percent, kind); kind 'cut' = a perturbed stream window,
+    'walk' = an independent random walk."""
+    rng = np.random.default_rng(seed)
+    qs, bands = [], []
+    for i, (m, p, kind) in enumerate(specs):
+        if kind == "cut":
+            q, _ = synth.cut_query(stream, m, seed + i)
+        else:
+            q = synth.random_walk(m, seed + 100 + i)
+        qs.append(O.znormalize_f32(q.astype(np.float32)))
+        bands.append(band_width(m, p))
+    return qs, np.array(bands, np.int32)
+
+
+SPECS = [(8, 20, "walk"), (16, 5, "cut"), (64, 10, "walk"), (100, 20, "cut"), (128, 10, "cut"),
+         (200, 5, "walk"), (256, 10, "cut"), (300, 40, "walk"), (333, 35, "cut"), (97, 0, "cut"),
+         (512, 10, "walk"), (40, 100, "cut")]
+
+
+@pytest.mark.parametrize("k", [1, 4])
+@pytest.mark.parametrize("metric", [METRIC_L2, METRIC_L1])
+def test_batch_equals_plain_search(engine, k, metric):
+    s = synth.ecg_like(1 << 16, 5)
+    ds = dev(s)
+    qs, bands = make_queries(s, SPECS, 17)
+    assert any(b > 112 for b in bands)  # the anti-diagonal kernel is covered too
+    b = engine.batch(qs, bands, pipelines=3)
+    r = b.search(ds, k=k, metric=metric)
+    for i, (q, w) in enumerate(zip(qs, bands)):
+        engine.set_query(q, int(w))
+        p = engine.search(ds, int(w), metric=metric, k=k, excl=q.size // 2)
+        np.testing.assert_array_equal(r.positions[i], p.positions, err_msg=f"query {i}")
+        np.testing.assert_array_equal(r.distances[i], p.distances, err_msg=f"query {i}")
+        assert r.infos[i]["found"] == int(np.sum(p.positions >= 0))
+    r2 = b.search(ds, k=k, metric=metric)  # graph replay
+    np.testing.assert_array_equal(r2.positions, r.positions)
+    np.testing.assert_array_equal(r2.distances, r.distances)
+    b.close()
+
+
+def test_batch_against_oracle_profile(engine):
+    s = synth.random_walk(1 << 13, 3)
+    ds = dev(s)
+    qs, bands = make_queries(s, [(24, 10, "cut"), (50, 20, "walk"), (77, 5, "cut")], 9)
+    b = engine.batch(qs, bands)
+    r = b.search(ds, k=3, excl=[12, 25, 38])
+    for i, (q, w) in enumerate(zip(qs, bands)):
+        # the oracle's DP over the GPU's own window statistics: bit-exact
+        avg, sd = engine.avg_std(ds, q.size)
+        torch.cuda.synchronize()
+        prof = O.cdtw(q, s, avg.cpu().numpy(), sd.cpu().numpy(), int(w))
+        idx, dist = O.topk_greedy(prof, 3, [12, 25, 38][i])
+        np.testing.assert_array_equal(r.positions[i, :idx.size], idx)
+        np.testing.assert_array_equal(r.distances[i, :idx.size], np.sqrt(dist))
+    b.close()
+
+
+def test_batch_tuned_shapes_keep_results(engine):
+    s = synth.ecg_like(1 << 15, 8)
+    ds = dev(s)
+    qs, bands = make_queries(s, SPECS[1:9], 23)
+    b = engine.batch(qs, bands, pipelines=2)
+    r0 = b.search(ds, k=2)
+    nb = b.tune(ds, k=2, cell_budget=1 << 30)
+    assert nb >= 3
+    r1 = b.search(ds, k=2)
+    np.testing.assert_array_equal(r1.positions, r0.positions)
+    np.testing.assert_array_equal(r1.distances, r0.distances)
+    b.close()
+
+
+def test_batch_argument_errors(engine):
+    from cudadtw_amd._lib import MdtwError
+    q = O.znormalize_f32(synth.random_walk(32, 1))
+    q2 = O.znormalize_f32(synth.random_walk(2048, 1))
+    with pytest.raises(MdtwError):
+        engine.batch([q2], [1500])         # band beyond the supported range
+    with pytest.raises(MdtwError):
+        engine.batch([q, q * np.nan], [3, 3])
+    b = engine.batch([q], [3])
+    with pytest.raises(MdtwError):
+        b.search(dev(np.zeros(16, np.float32)))  # stream shorter than the query
+    with pytest.raises(MdtwError):
+        b.search(dev(synth.random_walk(1000, 2)), k=0)
+    b.close()
