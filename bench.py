@@ -1,22 +1,33 @@
-"""Benchmark: cDTW GCUPS and windows/s on one MI355X.
+"""Benchmark: cDTW GCUPS and windows/s on one MI355X; batched and backward vs
+plain on the same GPU.
 
-Workload (one step): an ECG-like FP32 stream of 2^27 samples (Gaussian-bump
-beats, period 250 +- 15, N(0, 0.05) noise, baseline wander), a 256-sample
-noisy-beat query, Sakoe-Chiba band W = 26 (10%): window statistics (FP64
-prefix sums), the banded cDTW of every one of the 134,217,473 windows, and
-the best-match argmin.  ``value`` is that step's throughput in GCUPS (band
-cells M(2W+1) - W(W+1) per window, all windows, divided by the step time),
-inputs resident in HBM.
+Main workload (one step, ``value``): an ECG-like FP32 stream of 2^27 samples
+(Gaussian-bump beats, period 250 +- 15, N(0, 0.05) noise, baseline wander),
+a 256-sample noisy-beat query, Sakoe-Chiba band W = 26 (10%): window
+statistics (FP64 prefix sums), the banded cDTW of every one of the
+134,217,473 windows, and the best-match argmin.  ``value`` is that step's
+throughput in GCUPS (band cells M(2W+1) - W(W+1) per window, all windows,
+divided by the step time), inputs resident in HBM.
 
-Also reported on the same line: the same step at W = 13 and 51, the pruned
-best-match search (windows/s and effective GCUPS, as the reference's
-GPU-DTW), z-normalised ED search windows/s, the dominant kernel's roofline
-(FP32 VALU) from HIP events on the stream it runs on, and a CPU baseline
-timed on this host: the restated reference CPU path (brute-force banded DTW
-over all windows, OpenMP, cdtw.cuh:90-157) on a bounded sample, plus the
-single-threaded UCR-cascade port.
+Also on the same line, one object per workload:
+  c2      the same step at W = 13 and 51; the pruned best-match search
+          (windows/s, effective GCUPS); z-normalised ED search,
+  c1      random-walk stream 2^20, walk query M = 128, W = 13 (best match),
+  c3      metal-like stream 2^26, cut queries M = 512 / 1024, W = 10 %,
+          top-16 with exclusion M/2, cDTW and LDTW,
+  c4      mixed batch: queries |Q| in [64, 1024], bands 5-20 %, top-8, one
+          captured-graph batch launch vs the same queries as one plain search
+          each (default: every 16th of the 1024 queries; --c4 full: all),
+  c5      256 queries |Q| in [128, 512], W = 10 %, 2^24 stream: top-8 forward,
+          query-gradient backward (recompute GCUPS), 50-step template
+          refinement,
+plus the dominant kernel's roofline (FP32 VALU) from HIP events on the
+stream it runs on, and a CPU baseline timed on this host (the restated
+reference CPU path, brute-force banded DTW over all windows with OpenMP,
+cdtw.cuh:90-157, on a bounded sample; and the single-threaded UCR-cascade
+port).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--c4 sample|full|off]
 """
 import argparse
 import json
@@ -71,39 +82,26 @@ def cpu_baseline(stream, zq, q_raw, m, w):
                                        f"port, {t_ucr:.2f} s, location {loc}"})
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--log2n", type=int, default=27)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="also run one 1 GiB device copy (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU, and the "
-              "search path has no multi-GPU code (see DESIGN.md for how it would shard).",
-              file=sys.stderr)
-        sys.exit(2)
+def timed(torch, fn, reps=1):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r = fn()
+    torch.cuda.synchronize()
+    return r, (time.perf_counter() - t0) / reps
 
-    import torch
 
-    from cudadtw_amd import ALGO_ED, Engine, synth
-    from cudadtw_amd import znormalize
+def section_c2(eng, torch, args):
+    from cudadtw_amd import ALGO_ED, synth, znormalize
 
     m, w = 256, 26
-    log(f"generating ECG-like stream 2^{args.log2n} ...")
-    t0 = time.time()
+    log(f"[c2] generating ECG-like stream 2^{args.log2n} ...")
     stream = synth.config2_stream(args.log2n, 0)
     q_raw = synth.ecg_query(m, 2)
     zq = znormalize(q_raw)
-    log(f"  {time.time() - t0:.1f} s")
     n = stream.size
     L = n - m + 1
-
-    eng = Engine(0)
-    name, cus = eng.info()
     ds = torch.from_numpy(stream).to("cuda:0")
     avg = eng.empty(L)
     sd = eng.empty(L)
@@ -145,32 +143,229 @@ def main():
                        "gcups": L * cells(m, wb) / dt / 1e9,
                        "dp_gcups": L * cells(m, wb) / (dp_ms / 1e3) / 1e9,
                        "best": [int(best[0]), float(np.sqrt(best[1]))]}
-        log(f"W={wb}: {dt*1e3:.2f} ms/step, {results[wb]['gcups']:.1f} GCUPS, DP kernel "
+        log(f"[c2] W={wb}: {dt*1e3:.2f} ms/step, {results[wb]['gcups']:.1f} GCUPS, DP kernel "
             f"{dp_ms:.2f} ms ({results[wb]['dp_gcups']:.1f} GCUPS), best {results[wb]['best']}")
 
-    # pruned best-match search (the reference's GPU-DTW path), stats included
     eng.set_query(zq, w)
-    r = eng.search(ds, w)
-    torch.cuda.synchronize()
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        r = eng.search(ds, w)
-    t_search = (time.perf_counter() - t0) / reps
-    log(f"search: {t_search*1e3:.2f} ms, location {r.location} distance {r.distance:.5f}, "
+    r, t_search = timed(torch, lambda: eng.search(ds, w), reps=5)
+    log(f"[c2] search: {t_search*1e3:.2f} ms, location {r.location} distance {r.distance:.5f}, "
         f"info {r.info}")
     assert r.location == results[w]["best"][0], "pruned search disagrees with the full profile"
+    re_, t_ed = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=5)
+    log(f"[c2] ED search: {t_ed*1e3:.2f} ms, location {re_.location}")
+    out = {
+        "main": results[w], "L": L, "n": n, "m": m, "w": w,
+        "gcups_by_band": {str(k): round(v["gcups"], 2) for k, v in results.items()},
+        "dp_kernel_gcups_by_band": {str(k): round(v["dp_gcups"], 2) for k, v in results.items()},
+        "pruned_search": {"ms": round(t_search * 1e3, 3), "windows_per_s": round(L / t_search, 1),
+                          "effective_gcups": round(L * cells(m, w) / t_search / 1e9, 2),
+                          "location": r.location, "distance": round(r.distance, 6),
+                          "dp_evaluated": r.info["dp_evaluated"],
+                          "lb_kim_pruned": r.info["lb_kim_pruned"],
+                          "lb_keogh_pruned": r.info["lb_keogh_pruned"]},
+        "ed_search": {"ms": round(t_ed * 1e3, 3), "windows_per_s": round(L / t_ed, 1),
+                      "location": re_.location},
+    }
+    return out, stream, q_raw, zq
 
-    # ED search
-    re_ = eng.search(ds, 0, algo=ALGO_ED)
+
+def section_c1(eng, torch):
+    from cudadtw_amd import synth, znormalize
+
+    s, q, w = synth.config1()
+    ds = torch.from_numpy(s).to("cuda:0")
+    eng.set_query(znormalize(q), w)
+    r, t = timed(torch, lambda: eng.search(ds, w), reps=5)
+    L = s.size - q.size + 1
+    log(f"[c1] search {t*1e3:.2f} ms, location {r.location} distance {r.distance:.5f}")
+    return {"workload": "random walk 2^20 (seed 0), walk query M=128 (seed 1), W=13, best match",
+            "ms": round(t * 1e3, 3), "windows_per_s": round(L / t, 1),
+            "effective_gcups": round(L * cells(128, w) / t / 1e9, 2),
+            "location": r.location, "distance": round(r.distance, 6)}
+
+
+def section_c3(eng, torch):
+    from cudadtw_amd import ALGO_CDTW, ALGO_LDTW, synth, znormalize
+
+    log("[c3] generating metal-like stream 2^26 ...")
+    s = synth.metal_like(1 << 26, 3)
+    ds = torch.from_numpy(s).to("cuda:0")
+    out = {"workload": "metal-like 2^26, cut queries (N(0,0.05)), W=10%, top-16, excl M/2"}
+    for m in (512, 1024):
+        q, pos0 = synth.cut_query(s, m, 30 + m)
+        w = int(np.floor(m * 0.1 + 0.5))
+        eng.set_query(znormalize(q), w)
+        avg, sd = eng.avg_std(ds, m)
+        L = s.size - m + 1
+        for name, algo in (("cdtw", ALGO_CDTW), ("ldtw", ALGO_LDTW)):
+            r, t = timed(torch, lambda: eng.search(ds, w, algo=algo, k=16, excl=m // 2,
+                                                   avg=avg, sd=sd), reps=2)
+            out[f"{name}_m{m}"] = {"w": w, "ms": round(t * 1e3, 3),
+                                   "windows_per_s": round(L / t, 1),
+                                   "first": int(r.positions[0]), "cut_at": pos0,
+                                   "found": int(r.info["found"])}
+            log(f"[c3] {name} M={m} W={w}: {t*1e3:.2f} ms, top match {r.positions[0]} "
+                f"(cut at {pos0})")
+    return out
+
+
+def section_c4(eng, torch, args):
+    from cudadtw_amd import synth, znormalize
+
+    log("[c4] generating ECG-like stream 2^26 and 1024 queries ...")
+    s, qs, bands, kinds = synth.config4()
+    sel = list(range(0, len(qs), 1 if args.c4 == "full" else args.c4_stride))
+    zqs = [znormalize(qs[i]) for i in sel]
+    bs = bands[sel]
+    ds = torch.from_numpy(s).to("cuda:0")
+    k = 8
+    Ls = np.array([s.size - z.size + 1 for z in zqs], np.float64)
+    cellsum = float(sum(L * cells(z.size, int(b)) for L, z, b in zip(Ls, zqs, bs)))
+    batch = eng.batch(zqs, bs, pipelines=4)
+    t0 = time.perf_counter()
+    nb = batch.tune(ds, k=k, cell_budget=1 << 38)
+    t_tune = time.perf_counter() - t0
+    log(f"[c4] tuned {nb} buckets in {t_tune:.1f} s; capturing and running the batch ...")
+    rb = batch.search(ds, k=k)  # captures the graph
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(reps):
-        re_ = eng.search(ds, 0, algo=ALGO_ED)
-    t_ed = (time.perf_counter() - t0) / reps
-    log(f"ED search: {t_ed*1e3:.2f} ms, location {re_.location}")
+    rb = batch.search(ds, k=k)
+    t_batch = time.perf_counter() - t0
+    log(f"[c4] batch of {len(zqs)}: {t_batch:.3f} s (graph device time {rb.elapsed_ms:.1f} ms)")
+    pos = np.full((len(zqs), k), -1, np.int64)
+    dist = np.full((len(zqs), k), np.inf, np.float32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (z, w) in enumerate(zip(zqs, bs)):
+        eng.set_query(z, int(w))
+        r = eng.search(ds, int(w), k=k, excl=z.size // 2)
+        pos[i], dist[i] = r.positions, r.distances
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    same = bool(np.array_equal(pos, rb.positions) and np.array_equal(dist, rb.distances))
+    log(f"[c4] plain, one search per query: {t_plain:.3f} s; speedup {t_plain / t_batch:.2f}x; "
+        f"identical results: {same}")
+    batch.close()
+    return {"workload": f"ECG-like 2^26, {len(zqs)} of 1024 queries "
+                        f"({'all' if args.c4 == 'full' else 'every %d-th' % args.c4_stride}), "
+                        "|Q| 64-1024, bands 5-20%, half cut+N(0,0.05), half walks, top-8, "
+                        "excl |Q|/2",
+            "queries": len(zqs), "wide_band_queries": int(np.sum(bs > 112)),
+            "batch_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
+            "speedup": round(t_plain / t_batch, 3), "identical": same,
+            "tune_s": round(t_tune, 2), "buckets_tuned": nb,
+            "windows_per_s_batch": round(float(Ls.sum()) / t_batch, 1),
+            "effective_gcups_batch": round(cellsum / t_batch / 1e9, 2)}
 
-    main_res = results[w]
+
+def section_c5(eng, torch, args):
+    from cudadtw_amd import synth, znormalize
+    from cudadtw_amd.autograd import topk_cdtw
+
+    log("[c5] generating ECG-like stream 2^24 and 256 queries ...")
+    s, qs, bands, _ = synth.config5()
+    ds = torch.from_numpy(s).to("cuda:0")
+    k = 8
+    fwd = []
+    t_stats = t_fwd = t_bwd = 0.0
+    rec_cells = 0
+    fwd_cells = 0.0
+    for q, w in zip(qs, bands):
+        m, w = q.size, int(w)
+        zq = znormalize(q)
+        eng.set_query(zq, w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        avg, sd = eng.avg_std(ds, m)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r = eng.search(ds, w, k=k, excl=m // 2, avg=avg, sd=sd)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        found = int(np.sum(r.positions >= 0))
+        idx = torch.from_numpy(r.positions[:found].astype(np.int32)).to("cuda:0")
+        g = torch.ones(found, dtype=torch.float32, device="cuda:0")
+        qd = torch.from_numpy(q).to("cuda:0")
+        gq, _ = eng.cdtw_backward(ds, avg, sd, idx, g, w, query_raw=qd)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        t_stats += t1 - t0
+        t_fwd += t2 - t1
+        t_bwd += t3 - t2
+        rec_cells += found * cells(m, w)
+        fwd_cells += (s.size - m + 1) * cells(m, w)
+        fwd.append(float(np.sum(r.distances[:found])))
+    log(f"[c5] stats {t_stats:.3f} s, forward top-{k} {t_fwd:.3f} s, backward {t_bwd:.3f} s "
+        f"({rec_cells / t_bwd / 1e9:.2f} GCUPS recompute)")
+    # template refinement (DBA-style): gradient steps on query 0's template
+    q0 = qs[0]
+    w0 = int(bands[0])
+    avg, sd = eng.avg_std(ds, q0.size)
+    qt = torch.tensor(q0, dtype=torch.float32, requires_grad=True)
+    opt = torch.optim.SGD([qt], lr=0.05)
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(50):
+        opt.zero_grad()
+        d, _ = topk_cdtw(qt, eng, ds, avg, sd, w0, k, q0.size // 2)
+        loss = d.sum()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+    t_dba = time.perf_counter() - t0
+    log(f"[c5] refinement: loss {losses[0]:.4f} -> {losses[-1]:.4f} in 50 steps, {t_dba:.2f} s")
+    return {"workload": "ECG-like 2^24, 256 queries |Q| 128-512, W=10%, loss = sum of top-8 "
+                        "distances, backward dLoss/dQuery",
+            "stats_s": round(t_stats, 4), "forward_s": round(t_fwd, 4),
+            "backward_s": round(t_bwd, 4),
+            "forward_effective_gcups": round(fwd_cells / t_fwd / 1e9, 2),
+            "backward_recompute_gcups": round(rec_cells / t_bwd / 1e9, 3),
+            "backward_over_forward": round(t_bwd / t_fwd, 4),
+            "refinement": {"steps": 50, "query": 0, "m": int(q0.size), "w": w0,
+                           "loss_first": round(losses[0], 5), "loss_last": round(losses[-1], 5),
+                           "seconds": round(t_dba, 3)}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2n", type=int, default=27)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="c2,c1,c3,c4,c5")
+    ap.add_argument("--c4", choices=["sample", "full", "off"], default="sample")
+    ap.add_argument("--c4-stride", type=int, default=16)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 1 GiB device copy (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every GPU box here has one GPU, and the "
+              "search path has no multi-GPU code (see DESIGN.md for how it would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+    sections = set(args.sections.split(","))
+
+    import torch
+
+    from cudadtw_amd import Engine
+
+    eng = Engine(0)
+    name, cus = eng.info()
+    m, w = 256, 26
+    extra = {}
+    c2, stream, q_raw, zq = section_c2(eng, torch, args)
+    main_res = c2.pop("main")
+    L = c2["L"]
+    if "c1" in sections:
+        extra["c1"] = section_c1(eng, torch)
+    if "c3" in sections:
+        extra["c3"] = section_c3(eng, torch)
+    if "c4" in sections and args.c4 != "off":
+        extra["c4"] = section_c4(eng, torch, args)
+    if "c5" in sections:
+        extra["c5"] = section_c5(eng, torch, args)
+
     flops_per_window = 5 * cells(m, w) + 2 * m
     achieved = L * flops_per_window / (main_res["dp_ms"] / 1e3) / 1e12
     out = {
@@ -189,24 +384,15 @@ def main():
         "config": {"workload": f"ecg_like stream 2^{args.log2n} samples, query M={m} noisy "
                                f"beat, band W={w} (10%): stats + cDTW of all {L} windows "
                                f"+ argmin",
-                   "n": n, "m": m, "w": w, "device": name, "cus": cus},
+                   "n": c2["n"], "m": m, "w": w, "device": name, "cus": cus},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                      "kernel": "band_kernel<26,true> (register-band cDTW)",
                      "flops_per_window": flops_per_window},
-        "gcups_by_band": {str(k): round(v["gcups"], 2) for k, v in results.items()},
-        "dp_kernel_gcups_by_band": {str(k): round(v["dp_gcups"], 2) for k, v in results.items()},
-        "pruned_search": {"ms": round(t_search * 1e3, 3),
-                          "windows_per_s": round(L / t_search, 1),
-                          "effective_gcups": round(L * cells(m, w) / t_search / 1e9, 2),
-                          "location": r.location, "distance": round(r.distance, 6),
-                          "dp_evaluated": r.info["dp_evaluated"],
-                          "lb_kim_pruned": r.info["lb_kim_pruned"],
-                          "lb_keogh_pruned": r.info["lb_keogh_pruned"]},
-        "ed_search": {"ms": round(t_ed * 1e3, 3), "windows_per_s": round(L / t_ed, 1),
-                      "location": re_.location},
+        "c2": {k: v for k, v in c2.items() if k not in ("L", "n", "m", "w")},
     }
+    out.update(extra)
     if not args.no_cpu:
         log("CPU baseline ...")
         cb, ucr = cpu_baseline(stream, zq, q_raw, m, w)

@@ -38,6 +38,11 @@ constexpr int kBandChunk = 16;
 constexpr int kBandWaves = 4;  // waves per workgroup
 // Narrow-band limit: query window plus prefetch held in scalar registers.
 constexpr int kBandSlideMax = 42;  // spill-free up to W = 17
+// Bands up to kBandMaxW unroll kBandU columns per block; wider bands (L2, up
+// to kBandWideMaxW) unroll one column, which keeps the 2W + 1 band values in
+// vector + accumulation registers (no scratch memory) at one wave per SIMD.
+constexpr int kBandMaxW = 112;
+constexpr int kBandWideMaxW = 240;
 
 __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane) {
     unsigned long long b = 0;
@@ -78,7 +83,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                                          const float* __restrict__ qx, int32_t m, bool& alive,
                                          int mode, const Ctrl* ctrl) {
     constexpr int B = 2 * W + 1;
-    constexpr int U = kBandU;
+    constexpr int U = W > kBandMaxW ? 1 : kBandU;
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
     // Narrow bands keep the query window of a whole block of U columns in
@@ -263,7 +268,9 @@ BandLaunchFn band_table_0(int w, bool l2);  // W 0..29
 BandLaunchFn band_table_1(int w, bool l2);  // W 30..59
 BandLaunchFn band_table_2(int w, bool l2);  // W 60..89
 BandLaunchFn band_table_3(int w, bool l2);  // W 90..kBandMaxW
-
-constexpr int kBandMaxW = 112;
+BandLaunchFn band_table_4(int w);           // L2, W 113..144
+BandLaunchFn band_table_5(int w);           // L2, W 145..176
+BandLaunchFn band_table_6(int w);           // L2, W 177..208
+BandLaunchFn band_table_7(int w);           // L2, W 209..kBandWideMaxW
 
 }  // namespace mdtw

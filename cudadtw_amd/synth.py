@@ -105,3 +105,43 @@ def config1():
 
 def config2_stream(log2n: int = 27, seed: int = 0):
     return ecg_like(1 << log2n, seed)
+
+
+def batch_queries(stream: np.ndarray, nq: int = 1024, seed: int = 4, mmin: int = 64,
+                  mmax: int = 1024, pmin: float = 5.0, pmax: float = 20.0):
+    """Mixed-shape query set: lengths uniform in [mmin, mmax], band percent
+    uniform in [pmin, pmax] (W = round(p * M / 100)); even-numbered queries
+    are stream windows perturbed with N(0, 0.05), odd ones independent
+    random walks.  Returns (raw queries, bands, kinds)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(mmin, mmax + 1, size=nq)
+    pct = rng.uniform(pmin, pmax, size=nq)
+    qs, bands, kinds = [], [], []
+    for i in range(nq):
+        m = int(lens[i])
+        if i % 2 == 0:
+            q, _ = cut_query(stream, m, seed * 100003 + i)
+            kinds.append("cut")
+        else:
+            q = random_walk(m, seed * 100003 + i)
+            kinds.append("walk")
+        qs.append(q)
+        bands.append(int(np.floor(m * pct[i] / 100.0 + 0.5)))
+    return qs, np.array(bands, np.int32), kinds
+
+
+def config4(log2n: int = 26, seed: int = 4):
+    """1024 queries, |Q| uniform in [64, 1024], bands 5-20 %, half cut from
+    and perturbed against the stream, half independent walks; one ECG-like
+    stream of 2^26 samples."""
+    s = ecg_like(1 << log2n, seed)
+    qs, bands, kinds = batch_queries(s, 1024, seed)
+    return s, qs, bands, kinds
+
+
+def config5(log2n: int = 24, seed: int = 5, nq: int = 256):
+    """256 queries, |Q| uniform in [128, 512], band 10 %; ECG-like stream of
+    2^24 samples (half cut, half walks)."""
+    s = ecg_like(1 << log2n, seed)
+    qs, bands, kinds = batch_queries(s, nq, seed, 128, 512, 10.0, 10.0)
+    return s, qs, bands, kinds

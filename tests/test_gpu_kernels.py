@@ -86,7 +86,9 @@ def test_stats_refcompat_matches_restatement(engine):
 # ---- cDTW profile: bit-exact with the restated sequential DP ------------------------
 @pytest.mark.parametrize("m,w", [(128, 13), (128, 0), (128, 1), (128, 26), (256, 51), (100, 29),
                                  (100, 30), (256, 59), (256, 60), (256, 89), (256, 90),
-                                 (256, 112), (37, 5), (11, 10), (5, 2), (3, 1), (2, 1)])
+                                 (256, 112), (37, 5), (11, 10), (5, 2), (3, 1), (2, 1),
+                                 (300, 113), (300, 144), (400, 145), (400, 177), (500, 208),
+                                 (500, 209), (600, 240), (700, 241)])
 def test_cdtw_bit_exact_random_order(engine, m, w):
     n = 6000
     s = synth.random_walk(n, m + w)
@@ -108,7 +110,7 @@ def test_cdtw_wide_kernel_bit_exact(engine, w):
     got = host(engine.gpu_cdtw(ds, avg, sd, w, wide=True))
     ref = O.cdtw(zq, s, ha, hs, w, fused=1)
     assert np.array_equal(got, ref), np.max(np.abs(got - ref))
-    if w > 112:  # the regular entry point dispatches there too
+    if w > 112:  # the regular entry point: register-band (W <= 240) or this kernel
         got2 = host(engine.gpu_cdtw(ds, avg, sd, w))
         assert np.array_equal(got2, ref)
 
