@@ -2,17 +2,17 @@
 //
 // Two modes (mdtw_avg_std, replacing stats.cuh:115-142 avg_std<double>):
 //
-// MDTW_STATS_FP64 (default).  One pass, no global prefix.  A workgroup owns T
-// consecutive windows [w0, w0+T).  With c = x[w0] as a tile-local shift,
-// window r's sums are
+// MDTW_STATS_FP64 (default).  One pass, no global prefix.  A wave owns a
+// tile of consecutive windows [w0, w0 + T).  With c = x[w0] as a tile-local
+// shift, window r's sums are
 //     S(r) = S0 + sum_{k<r} (x[w0+m+k] - x[w0+k])         (and the same for
 //                                                         squares of x - c),
-// S0 being the first window's sum.  S0 is a workgroup reduction, the
-// differences are a workgroup exclusive scan, all in FP64 on values shifted
-// by c, so cancellation is bounded by the tile's local range instead of the
-// stream's absolute level (the reference's global FP64 prefix loses ~1e-2 of
-// sigma on offset data; its FP32 finalize loses more, stats.cuh:40-52).
-// HBM traffic: read (T + m)/T floats per window, write 8 bytes.
+// S0 being the first window's sum, all in FP64 on values shifted by c, so
+// cancellation is bounded by the tile's local range instead of the stream's
+// absolute level (the reference's global FP64 prefix loses ~1e-2 of sigma on
+// offset data; its FP32 finalize loses more, stats.cuh:40-52).
+// HBM traffic: read (2T + m)/T floats per window (the second read of each
+// sample mostly hits L2), write 8 bytes.
 //
 // MDTW_STATS_REFCOMPAT.  Restates the reference arithmetic: per 10^7-sample
 // epoch with M-1 overlap (stats.cuh:120-141), a global inclusive FP64 prefix
@@ -66,75 +66,165 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     return t;
 }
 
-template <int R>
-__global__ __launch_bounds__(kThreads) void stats_fp64_kernel(
+// ---- MDTW_STATS_FP64: wave-local tiles -------------------------------------
+// A wave owns kStatsTW consecutive windows [w0, w0 + kStatsTW) and needs no
+// barrier: the first window's sums are a wave reduction (coalesced loads),
+// x[w0 + i] and x[w0 + m + i] for the tile are staged through LDS with
+// coalesced loads, lane l takes the kStatsR consecutive windows l*R .. l*R+R-1,
+// the transitions before them are a wave exclusive scan, and the results go
+// back through LDS for coalesced stores.  LDS index i is stored at
+// i + i/16, so lane l's consecutive elements (17l + v) fall in distinct banks.
+constexpr int kStatsR = 16;
+constexpr int kStatsTW = kWave * kStatsR;
+constexpr int kStatsWaves = 4;
+constexpr int kStatsLds = kStatsTW + kStatsTW / 16;
+
+__device__ __forceinline__ int stats_skew(int i) { return i + (i >> 4); }
+
+// Wave-wide inclusive scan of a double through DPP (GFX9 family): row_shr
+// 1/2/4/8 inside 16-lane rows, then row_bcast:15 and row_bcast:31 carry row
+// totals upward.  Register-only; lanes with no DPP source add 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROWS, 0xf, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ double wave_incl_scan_f64(double v) {
+    v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    const double incl = wave_incl_scan_f64(v);
+    const long long b = __double_as_longlong(incl);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), kWave - 1);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), kWave - 1);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+
+__device__ __forceinline__ double wave_excl_scan_f64(double v) {
+    return wave_incl_scan_f64(v) - v;
+}
+
+__device__ __forceinline__ void stats_wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
     const float* __restrict__ s, int64_t L, int32_t m, float* __restrict__ avg,
     float* __restrict__ sd) {
-    constexpr int T = kThreads * R;
-    __shared__ double sh[kThreads / kWave];
-    const int64_t w0 = static_cast<int64_t>(blockIdx.x) * T;
-    const int tid = threadIdx.x;
-    const int64_t nw64 = L - w0;
-    const int nw = nw64 < T ? static_cast<int>(nw64) : T;
+    __shared__ float xa_all[kStatsWaves][kStatsLds];
+    __shared__ float xb_all[kStatsWaves][kStatsLds];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    float* xa = xa_all[wid];
+    float* xb = xb_all[wid];
+    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
+    if (w0 >= L) return;
+    const int nw = L - w0 < kStatsTW ? static_cast<int>(L - w0) : kStatsTW;
+    const int64_t last = L - 1 + m - 1;  // last stream sample
     const float* x = s + w0;
     const double c = static_cast<double>(x[0]);
 
-    // S0: first window of the tile.
+    // Stage x[w0 + i] (leaving sample of window i) and x[w0 + m + i] (entering);
+    // every load of the tile is issued before any is used.
+    float va[kStatsR], vb[kStatsR];
+#pragma unroll
+    for (int v = 0; v < kStatsR; ++v) {
+        const int i = v * kWave + lane;
+        const int64_t ga = w0 + i, gb = w0 + m + i;
+        va[v] = s[ga < last ? ga : last];
+        vb[v] = s[gb < last ? gb : last];
+    }
+#pragma unroll
+    for (int v = 0; v < kStatsR; ++v) {
+        const int i = v * kWave + lane;
+        xa[stats_skew(i)] = va[v];
+        xb[stats_skew(i)] = vb[v];
+    }
+    stats_wave_fence();
+
+    // First window of the tile: x[w0 .. w0 + m), from the staged tile where
+    // it lies inside it (x[w0 + k] = xa[k] for k < kStatsTW).
     double a1 = 0.0, a2 = 0.0, b1 = 0.0;
-    for (int k = tid; k < m; k += kThreads) {
-        double d = static_cast<double>(x[k]) - c;
+    for (int k = lane; k < m; k += kWave) {
+        const float xk = k < kStatsTW ? xa[stats_skew(k)] : x[k];
+        const double d = static_cast<double>(xk) - c;
         a1 += d;
         b1 += fabs(d);
         a2 = fma(d, d, a2);
     }
-    const double S1 = block_sum(a1, sh);
-    const double S2 = block_sum(a2, sh);
-    const double B1 = block_sum(b1, sh);
+    const double S1 = wave_sum_f64(a1);
+    const double S2 = wave_sum_f64(a2);
+    const double B1 = wave_sum_f64(b1);
 
-    // Transition terms of this thread's R windows (k = tid*R + v), used by
-    // windows k+1 and later.
-    double d1[R], d2[R], e1[R], e2[R];
+    // Pass 1: this lane's transition totals (window r -> r + 1).
+    const int r0 = lane * kStatsR;
     double t1 = 0.0, t2 = 0.0, u1 = 0.0, u2 = 0.0;
-#pragma unroll
-    for (int v = 0; v < R; ++v) {
-        const int k = tid * R + v;
-        double in1 = 0.0, in2 = 0.0, ab1 = 0.0, ab2 = 0.0;
-        if (k + 1 < nw) {
-            double xo = static_cast<double>(x[k]) - c;
-            double xn = static_cast<double>(x[k + m]) - c;
-            in1 = xn - xo;
-            double xo2 = xo * xo, xn2 = xn * xn;
-            in2 = xn2 - xo2;
-            ab1 = fabs(xn) + fabs(xo);
-            ab2 = xn2 + xo2;
+    for (int v = 0; v < kStatsR; ++v) {
+        const int r = r0 + v;
+        if (r + 1 < nw) {
+            const double xo = static_cast<double>(xa[stats_skew(r)]) - c;
+            const double xn = static_cast<double>(xb[stats_skew(r)]) - c;
+            t1 += xn - xo;
+            t2 += xn * xn - xo * xo;
+            u1 += fabs(xn) + fabs(xo);
+            u2 += xn * xn + xo * xo;
         }
-        // exclusive prefix within the thread
-        d1[v] = t1; d2[v] = t2; e1[v] = u1; e2[v] = u2;
-        t1 += in1; t2 += in2; u1 += ab1; u2 += ab2;
     }
-    double tot;
-    const double p1 = block_scan_incl(t1, sh, &tot) - t1;
-    const double p2 = block_scan_incl(t2, sh, &tot) - t2;
-    const double q1 = block_scan_incl(u1, sh, &tot) - u1;
-    const double q2 = block_scan_incl(u2, sh, &tot) - u2;
+    double p1 = wave_excl_scan_f64(t1);
+    double p2 = wave_excl_scan_f64(t2);
+    double q1 = wave_excl_scan_f64(u1);
+    double q2 = wave_excl_scan_f64(u2);
 
+    // Pass 2: the lane's windows; results overwrite their own LDS slots
+    // after the slot's samples are read.
     const double inv_m = 1.0 / static_cast<double>(m);
-#pragma unroll
-    for (int v = 0; v < R; ++v) {
-        const int r = tid * R + v;
-        if (r < nw) {
-            const double sum1 = S1 + (p1 + d1[v]);
-            const double sum2 = S2 + (p2 + d2[v]);
-            const double mean = sum1 * inv_m;
-            double var = fma(-mean, mean, sum2 * inv_m);
-            // Rounding-level variance of an exactly constant window is noise:
-            // treat it as zero (sigma := 1, the reference's guard).
-            const double mag2 = (S2 + q2 + e2[v]) * inv_m;
-            const double mag1 = (B1 + q1 + e1[v]) * inv_m;
-            const double tol = 64.0 * DBL_EPSILON * (mag2 + 2.0 * fabs(mean) * mag1);
-            if (var <= tol) var = 0.0;
-            avg[w0 + r] = static_cast<float>(c + mean);
-            sd[w0 + r] = var > 0.0 ? static_cast<float>(sqrt(var)) : 1.0f;
+    for (int v = 0; v < kStatsR; ++v) {
+        const int r = r0 + v;
+        if (r >= nw) break;
+        const double sum1 = S1 + p1;
+        const double sum2 = S2 + p2;
+        const double mean = sum1 * inv_m;
+        double var = fma(-mean, mean, sum2 * inv_m);
+        // Rounding-level variance of an exactly constant window is noise:
+        // treat it as zero (sigma := 1, the reference's guard).
+        const double mag2 = (S2 + q2) * inv_m;
+        const double mag1 = (B1 + q1) * inv_m;
+        const double tol = 64.0 * DBL_EPSILON * (mag2 + 2.0 * fabs(mean) * mag1);
+        if (var <= tol) var = 0.0;
+        const int k = stats_skew(r);
+        const double xo = static_cast<double>(xa[k]) - c;
+        const double xn = static_cast<double>(xb[k]) - c;
+        xa[k] = static_cast<float>(c + mean);
+        // FP32 square root of the FP64 variance rounded to FP32: within one
+        // ulp of the FP64 root's rounding, at a fraction of its cost.
+        xb[k] = var > 0.0 ? sqrtf(static_cast<float>(var)) : 1.0f;
+        p1 += xn - xo;
+        p2 += xn * xn - xo * xo;
+        q1 += fabs(xn) + fabs(xo);
+        q2 += xn * xn + xo * xo;
+    }
+    stats_wave_fence();
+#pragma unroll 4
+    for (int v = 0; v < kStatsR; ++v) {
+        const int i = v * kWave + lane;
+        if (i < nw) {  // streamed out: not re-read by this kernel
+            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
+            __builtin_nontemporal_store(xb[stats_skew(i)], sd + w0 + i);
         }
     }
 }
@@ -246,17 +336,10 @@ int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m, floa
                       float* sd) {
     const int64_t L = n - m + 1;
     if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
-    if (m >= 2048) {
-        constexpr int T = kThreads * 16;
-        const int64_t nb = (L + T - 1) / T;
-        hipLaunchKernelGGL(stats_fp64_kernel<16>, dim3(static_cast<unsigned>(nb)),
-                           dim3(kThreads), 0, st, s, L, m, avg, sd);
-    } else {
-        constexpr int T = kThreads * 8;
-        const int64_t nb = (L + T - 1) / T;
-        hipLaunchKernelGGL(stats_fp64_kernel<8>, dim3(static_cast<unsigned>(nb)),
-                           dim3(kThreads), 0, st, s, L, m, avg, sd);
-    }
+    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
+    const int64_t nb = (L + per_block - 1) / per_block;
+    hipLaunchKernelGGL(stats_fp64_kernel, dim3(static_cast<unsigned>(nb)),
+                       dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);
