@@ -216,12 +216,14 @@ __global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
     const float* __restrict__ q, const float* __restrict__ lo, const float* __restrict__ up,
     int32_t m, int64_t wbase, int64_t L, bool use_best, Ctrl* ctrl,
-    int32_t* __restrict__ cand) {
+    int32_t* __restrict__ cand, float* __restrict__ cand_lb) {
     __shared__ float ring_all[kCascadeWaves][kRing];
     __shared__ int32_t cbuf_all[kCascadeWaves][kCandBuf];
+    __shared__ float lbuf_all[kCascadeWaves][kCandBuf];
     const int lane = threadIdx.x & (kWave - 1);
     float* ring = ring_all[threadIdx.x / kWave];
     int32_t* cbuf = cbuf_all[threadIdx.x / kWave];
+    float* lbuf = lbuf_all[threadIdx.x / kWave];
     int32_t nbuf = 0;
     auto flush = [&]() {
         wave_lds_fence();
@@ -229,7 +231,10 @@ __global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
         if (lane == 0)
             basec = atomicAdd(&ctrl->count, static_cast<unsigned long long>(nbuf));
         basec = __shfl(basec, 0, kWave);
-        for (int32_t t = lane; t < nbuf; t += kWave) cand[basec + t] = cbuf[t];
+        for (int32_t t = lane; t < nbuf; t += kWave) {
+            cand[basec + t] = cbuf[t];
+            if (cand_lb) cand_lb[basec + t] = lbuf[t];
+        }
         wave_lds_fence();
         nbuf = 0;
     };
@@ -294,7 +299,10 @@ __global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
         const unsigned long long km = __ballot(keep);
         n_keogh += __popcll(__ballot(alive && !keep));
         if (km) {
-            if (keep) cbuf[nbuf + lane_rank(km)] = static_cast<int32_t>(w);
+            if (keep) {
+                cbuf[nbuf + lane_rank(km)] = static_cast<int32_t>(w);
+                lbuf[nbuf + lane_rank(km)] = acc;
+            }
             nbuf += __popcll(km);
             if (nbuf > kCandBuf - kWave) flush();
         }
@@ -348,7 +356,7 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg, const floa
 int launch_cascade(hipStream_t st, const float* s, const float* avg, const float* sd,
                    const float* q, const float* lo, const float* up, int32_t m, int64_t base,
                    int64_t L, int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
-                   int64_t max_blocks) {
+                   float* cand_lb, int64_t max_blocks) {
     if (L <= 0) return MDTW_OK;
     // Enough waves to cover the chip several times over; each wave strides.
     const int64_t waves_needed = (L + kWave - 1) / kWave;
@@ -357,10 +365,10 @@ int launch_cascade(hipStream_t st, const float* s, const float* avg, const float
     const unsigned nb = static_cast<unsigned>(nb64);
     if (metric)
         hipLaunchKernelGGL(cascade_kernel<true>, dim3(nb), dim3(256), 0, st, s, avg, sd, q, lo,
-                           up, m, base, L, use_best, ctrl, cand);
+                           up, m, base, L, use_best, ctrl, cand, cand_lb);
     else
         hipLaunchKernelGGL(cascade_kernel<false>, dim3(nb), dim3(256), 0, st, s, avg, sd, q,
-                           lo, up, m, base, L, use_best, ctrl, cand);
+                           lo, up, m, base, L, use_best, ctrl, cand, cand_lb);
     return check_launch();
 }
 

@@ -78,10 +78,16 @@ __device__ __forceinline__ void band_column(float (&E)[B], const Q& q, float zj)
     }
 }
 
-template <int W, bool L2>
+// LB: cumulative-bound abandon (see suffix_limit): the DP also sums the
+// window's LB_Keogh terms over the columns it has done, with the envelope
+// held lane-distributed (lane l: lo/up[64c + l]) and broadcast per column by
+// readlane, the next 64 values loaded a chunk ahead.
+template <int W, bool L2, bool LB>
 __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, float sg,
                                          const float* __restrict__ qx, int32_t m, bool& alive,
-                                         int mode, const Ctrl* ctrl) {
+                                         int mode, const Ctrl* ctrl,
+                                         const float* __restrict__ elo,
+                                         const float* __restrict__ eup, float lbt) {
     constexpr int B = 2 * W + 1;
     constexpr int U = W > kBandMaxW ? 1 : kBandU;
     constexpr int CH = kBandChunk;
@@ -115,10 +121,40 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     }
 
     const bool abandon = mode == kModeBest || mode == kModeCollect;
+    const int lane = threadIdx.x & (kWave - 1);
+    float lo_c = 0.0f, up_c = 0.0f, lo_n = 0.0f, up_n = 0.0f, P = 0.0f;
+    if constexpr (LB) {
+        lo_c = elo[lane < m ? lane : m - 1];
+        up_c = eup[lane < m ? lane : m - 1];
+        lo_n = elo[kWave + lane < m ? kWave + lane : m - 1];
+        up_n = eup[kWave + lane < m ? kWave + lane : m - 1];
+    }
     for (int32_t j0 = 0; j0 < m; j0 += U) {
         float z[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) z[u] = band_z(xv[u], mu, sg, y, fast);
+        if constexpr (LB) {
+            // U divides 64, so a block of U columns never straddles a chunk.
+            if (j0 > 0 && (j0 & (kWave - 1)) == 0) {
+                lo_c = lo_n;
+                up_c = up_n;
+                const int32_t c = j0 + kWave + lane;
+                lo_n = elo[c < m ? c : m - 1];
+                up_n = eup[c < m ? c : m - 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (j0 + u < m) {
+                    const int sl = (j0 + u) & (kWave - 1);
+                    const float l = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(lo_c), sl));
+                    const float h = __int_as_float(
+                        __builtin_amdgcn_readlane(__float_as_int(up_c), sl));
+                    const float e = z[u] - __builtin_amdgcn_fmed3f(z[u], l, h);
+                    P = L2 ? fmaf(e, e, P) : P + fabsf(e);
+                }
+            }
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int32_t c = j0 + U + u;
@@ -180,7 +216,13 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
 #pragma unroll
             for (int t = 1; t < B; ++t) cm = fminf(cm, E[t]);
             const float thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
-            alive = alive && !(cm > thr);
+            if constexpr (LB) {
+                const float rest = lbt - P;
+                const float bound = cm + (rest > 0.0f ? rest : 0.0f);
+                alive = alive && !(cm > thr) && !(bound > suffix_limit(thr, lbt, m, L2));
+            } else {
+                alive = alive && !(cm > thr);
+            }
             if (!__any(alive)) return kInf;
         }
     }
@@ -192,11 +234,13 @@ __device__ __forceinline__ uint32_t band_lane_rank(unsigned long long mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-template <int W, bool L2>
+template <int W, bool L2, bool LB>
 __global__ __launch_bounds__(256) void band_kernel(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
     const float* __restrict__ qx, const int32_t* __restrict__ idx, float* __restrict__ out,
-    unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, BandScalars a) {
+    unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, BandScalars a,
+    const float* __restrict__ lbt, const float* __restrict__ elo,
+    const float* __restrict__ eup) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     int64_t count = a.count;
@@ -218,7 +262,8 @@ __global__ __launch_bounds__(256) void band_kernel(
         const int64_t w = idx ? static_cast<int64_t>(idx[gc]) : a.base + gc * a.stride;
         const float mu = avg[w], sg = sd[w];
         bool alive = valid;
-        const float d = band_dp<W, L2>(s + w, mu, sg, qx, a.m, alive, a.mode, ctrl);
+        const float d = band_dp<W, L2, LB>(s + w, mu, sg, qx, a.m, alive, a.mode, ctrl, elo,
+                                           eup, LB ? lbt[gc] : 0.0f);
         if (a.mode == kModeProfile || a.mode == kModeSeed) {
             if (valid) out[g] = d;
         }
@@ -249,12 +294,46 @@ __global__ __launch_bounds__(256) void band_kernel(
     }
 }
 
+// Waves per SIMD a kernel can keep resident, from its register count
+// (512 registers per lane per SIMD, allocated in granules of 8).
+inline int resident_waves(const void* fn) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, fn) != hipSuccess || attr.numRegs <= 0) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const int regs = (attr.numRegs + 7) & ~7;
+    return regs > 0 ? 512 / regs : 0;
+}
+
+// The cumulative-bound variant holds a few more registers; it is used unless
+// that drops residency below min(2, the plain variant's) waves per SIMD, where
+// the lost latency hiding costs more than the earlier abandon saves.
+template <int W, bool L2>
+bool band_lb_pays() {
+    static const bool pays = [] {
+        const int with = resident_waves(reinterpret_cast<const void*>(&band_kernel<W, L2, true>));
+        const int without =
+            resident_waves(reinterpret_cast<const void*>(&band_kernel<W, L2, false>));
+        return with > 0 && with >= (without < 2 ? without : 2);
+    }();
+    return pays;
+}
+
 template <int W, bool L2>
 int band_launch(hipStream_t st, unsigned grid, const BandArgs& a) {
     BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent,
                    0, a.qx_len};
-    hipLaunchKernelGGL((band_kernel<W, L2>), dim3(grid), dim3(kBandWaves * kWave), 0, st, a.s,
-                       a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc);
+    const bool lb = a.lbt && a.idx && a.lo && a.up &&
+                    (a.mode == kModeBest || a.mode == kModeCollect) && band_lb_pays<W, L2>();
+    if (lb)
+        hipLaunchKernelGGL((band_kernel<W, L2, true>), dim3(grid), dim3(kBandWaves * kWave), 0,
+                           st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, a.lbt,
+                           a.lo, a.up);
+    else
+        hipLaunchKernelGGL((band_kernel<W, L2, false>), dim3(grid), dim3(kBandWaves * kWave), 0,
+                           st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, nullptr,
+                           nullptr, nullptr);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

@@ -149,6 +149,12 @@ struct BandArgs {
     Ctrl* ctrl = nullptr;
     unsigned long long* res = nullptr;
     float* out = nullptr;
+    // Cumulative-bound abandon (register-band kernel, idx lists from the
+    // cascade): lbt[g] = the window's LB_Keogh total, lo/up = the query
+    // envelope.  NULL lbt: column-minimum abandon only.
+    const float* lbt = nullptr;
+    const float* lo = nullptr;
+    const float* up = nullptr;
 };
 
 struct BandScalars {
@@ -233,11 +239,12 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg,
 // Search cascade: LB_Kim then LB_Keogh on windows base..base+count-1;
 // survivors (bound not above the limit of ctrl->best, or of ctrl->thr) are
 // appended to cand (ctrl->count).
+// cand_lb (may be NULL) gets each survivor's LB_Keogh total.
 int launch_cascade(hipStream_t st, const float* s, const float* avg,
                    const float* sd, const float* q, const float* lo,
                    const float* up, int32_t m, int64_t base, int64_t count,
                    int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
-                   int64_t max_blocks = 256 * 64);
+                   float* cand_lb = nullptr, int64_t max_blocks = 256 * 64);
 // cdtw.hip: band kernel (qx = padded query) or, for wide bands, the
 // anti-diagonal kernel (q_plain).
 int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain,
@@ -327,6 +334,20 @@ __device__ __forceinline__ float prune_limit(float thr, int32_t m, bool l2) {
     const float rel = thr * (float)(3 * m + 8) * 6.0e-8f;
     const float absl = l2 ? 2.5e-7f * sqrtf((float)m * thr) : 1.2e-7f * (float)m;
     return thr + rel + absl + 1e-30f;
+}
+
+// Limit for the DP's cumulative test  colmin_j + max(0, T - P_j) > limit,
+// where T is the window's LB_Keogh total from the cascade and P_j the same
+// terms summed by the DP over columns <= j: T - P_j bounds the cost of the
+// columns still to come (each later column holds a path cell, whose cost is
+// at least that column's LB_Keogh term).  On top of prune_limit: the
+// cancellation in T - P_j (both sums of <= m rounded terms) and the two
+// normalisations (reciprocal in the cascade, quotient in the DP), each bounded
+// as in prune_limit.
+__device__ __forceinline__ float suffix_limit(float thr, float T, int32_t m, bool l2) {
+    const float rel = T * (float)(2 * m + 2) * 6.0e-8f;
+    const float absl = l2 ? 5.0e-7f * sqrtf((float)m * T) : 2.4e-7f * (float)m;
+    return prune_limit(thr, m, l2) + rel + absl;
 }
 
 }  // namespace mdtw

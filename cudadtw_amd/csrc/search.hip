@@ -153,7 +153,7 @@ size_t search_ws_bytes(int64_t L, int32_t k, int64_t excl, bool own_stats) {
     int64_t max_round = 0;
     for (const Round& r : plan_rounds(L, topk)) max_round = std::max(max_round, r.count);
     return carve_size({own_stats ? sizeof(float) * L : 0, own_stats ? sizeof(float) * L : 0,
-                       sizeof(int32_t) * max_round,
+                       sizeof(int32_t) * max_round, sizeof(float) * max_round,
                        topk ? sizeof(unsigned long long) * L : 0, sizeof(float) * sp.count,
                        topk ? sizeof(uint32_t) * tl.nb : 0,
                        topk ? sizeof(uint32_t) * tighten_ws_words() : 0});
@@ -174,6 +174,7 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     float* avg = own_stats ? cv.take<float>(L) : const_cast<float*>(p.d_avg);
     float* sd = own_stats ? cv.take<float>(L) : const_cast<float*>(p.d_std);
     int32_t* cand = cv.take<int32_t>(max_round);
+    float* cand_lb = cv.take<float>(max_round);
     unsigned long long* res = topk ? cv.take<unsigned long long>(L) : nullptr;
     float* seeds = cv.take<float>(sp.count);
     uint32_t* bmin = topk ? cv.take<uint32_t>(tl.nb) : nullptr;
@@ -231,10 +232,13 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
             rc = launch_round_reset(st, p.ctrl);
             if (rc == MDTW_OK)
                 rc = launch_cascade(st, p.s, avg, sd, p.q, p.lo, p.up, m, r.base, r.count,
-                                    p.metric, !topk, p.ctrl, cand, cascade_cap);
+                                    p.metric, !topk, p.ctrl, cand, cand_lb, cascade_cap);
             if (rc == MDTW_OK) {
                 BandArgs da = a;
                 da.idx = cand;
+                da.lbt = cand_lb;
+                da.lo = p.lo;
+                da.up = p.up;
                 da.idx_count_from_ctrl = 1;
                 da.persistent = 1;
                 da.mode = topk ? kModeCollect : kModeBest;
