@@ -17,7 +17,7 @@ Also on the same line, one object per workload:
           top-16 with exclusion M/2, cDTW and LDTW,
   c4      mixed batch: queries |Q| in [64, 1024], bands 5-20 %, top-8, one
           captured-graph batch launch vs the same queries as one plain search
-          each (default: every 16th of the 1024 queries; --c4 full: all),
+          each (default: every 32nd of the 1024 queries; --c4 full: all),
   c5      256 queries |Q| in [128, 512], W = 10 %, 2^24 stream: top-8 forward,
           query-gradient backward (recompute GCUPS), 50-step template
           refinement,
@@ -335,7 +335,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="c2,c1,c3,c4,c5")
     ap.add_argument("--c4", choices=["sample", "full", "off"], default="sample")
-    ap.add_argument("--c4-stride", type=int, default=16)
+    ap.add_argument("--c4-stride", type=int, default=32)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -368,6 +368,15 @@ def main():
 
     flops_per_window = 5 * cells(m, w) + 2 * m
     achieved = L * flops_per_window / (main_res["dp_ms"] / 1e3) / 1e12
+    # HBM bytes per window of this kernel at this shape, measured by rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes (calibrated against a device copy).
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "dp_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("m") == m and t.get("w") == w:
+            traffic = round(t["bytes_per_window"] * L)
     out = {
         "metric": "cDTW GCUPS + windows/s at 1 GPU; batched & bwd vs plain, same GPU",
         "value": round(main_res["gcups"], 2),
@@ -387,7 +396,8 @@ def main():
                    "n": c2["n"], "m": m, "w": w, "device": name, "cus": cus},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "traffic_unit": "bytes per launch (HBM, rocprofv3 PMC)",
                      "kernel": "band_kernel<26,true> (register-band cDTW)",
                      "flops_per_window": flops_per_window},
         "c2": {k: v for k, v in c2.items() if k not in ("L", "n", "m", "w")},

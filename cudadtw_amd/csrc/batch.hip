@@ -150,7 +150,7 @@ struct mdtw_batch {
         p.q = d_arena + off[i];
         p.lo = p.q + m[i];
         p.up = p.lo + m[i];
-        p.qx = w[i] <= kBandMaxW ? d_arena + qx_off[i] : nullptr;
+        p.qx = d_arena + qx_off[i];  // launch_dp picks the kernel for (w, metric)
         p.qx_len = qx_len[i];
         p.m = m[i];
         p.w = w[i];
@@ -215,9 +215,15 @@ int capture(mdtw_batch* b, const float* s, int64_t n, int32_t metric, int32_t k,
     std::vector<int32_t> order(b->nq);
     std::iota(order.begin(), order.end(), 0);
     std::vector<double> cost(b->nq);
-    for (int32_t i = 0; i < b->nq; ++i)
-        cost[i] = static_cast<double>(n - b->m[i] + 1) * band_cells(b->m[i], b->w[i]) *
-                  (b->w[i] > kBandMaxW ? 6.0 : 1.0);
+    // Relative cell rates of the kernels a band lands on (register band with
+    // four- or one-column unroll, anti-diagonal).
+    for (int32_t i = 0; i < b->nq; ++i) {
+        const int32_t w = b->w[i];
+        const double rate = w <= kBandMaxW ? 1.0
+                            : (metric == MDTW_METRIC_L2 && w <= kBandWideMaxW) ? 2.2
+                                                                             : 6.5;
+        cost[i] = static_cast<double>(n - b->m[i] + 1) * band_cells(b->m[i], w) * rate;
+    }
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t c) { return cost[a] > cost[c]; });
     std::vector<double> load(P, 0.0);
