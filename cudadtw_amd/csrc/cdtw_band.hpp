@@ -36,11 +36,15 @@ constexpr int kBandU = 4;
 constexpr int kBandCheck = 16;
 constexpr int kBandChunk = 16;
 constexpr int kBandWaves = 4;  // waves per workgroup
-// Narrow-band limit: query window plus prefetch held in scalar registers.
-constexpr int kBandSlideMax = 42;  // spill-free up to W = 17
-// Bands up to kBandMaxW unroll kBandU columns per block; wider bands (L2, up
-// to kBandWideMaxW) unroll one column, which keeps the 2W + 1 band values in
-// vector + accumulation registers (no scratch memory) at one wave per SIMD.
+// Narrow-band limit: query window plus prefetch held in scalar registers
+// (up to W = 23; measured over W = 0..112 against the chunked loads, which
+// stall on their scalar waits for W = 18..23 and are faster from W = 24).
+constexpr int kBandSlideMax = 54;
+// Bands up to kBandMaxW unroll kBandU columns per block (W = 108..111: two,
+// W = 112: one, which keeps them within 256 registers, i.e. two waves per
+// SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which keeps
+// the 2W + 1 band values in vector + accumulation registers (no scratch
+// memory) at one wave per SIMD.
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 
@@ -89,7 +93,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                                          const float* __restrict__ elo,
                                          const float* __restrict__ eup, float lbt) {
     constexpr int B = 2 * W + 1;
-    constexpr int U = W > kBandMaxW ? 1 : kBandU;
+    constexpr int U = W > 111 ? 1 : (W > 107 ? 2 : kBandU);
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
     // Narrow bands keep the query window of a whole block of U columns in
