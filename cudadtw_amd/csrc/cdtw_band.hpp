@@ -128,6 +128,10 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     const int lane = threadIdx.x & (kWave - 1);
     float lo_c = 0.0f, up_c = 0.0f, lo_n = 0.0f, up_n = 0.0f, P = 0.0f;
     if constexpr (LB) {
+        // The bound may have fallen since the cascade kept this window.
+        const float thr0 = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
+        alive = alive && !(lbt > prune_limit(thr0, m, L2));
+        if (!__any(alive)) return kInf;
         lo_c = elo[lane < m ? lane : m - 1];
         up_c = eup[lane < m ? lane : m - 1];
         lo_n = elo[kWave + lane < m ? kWave + lane : m - 1];
