@@ -89,6 +89,8 @@ SIGNATURES = {
     "mdtw_batch_tune": (C.c_int, [_P, _P, _I64, _I32, _I32, _I64, C.POINTER(_I32)]),
     "mdtw_batch_search": (C.c_int, [_P, _P, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64),
                                     _FP, C.POINTER(SearchInfo), _FP]),
+    "mdtw_batch_backward": (C.c_int, [_P, _P, _I64, _I32, _I32, C.POINTER(_I64), _FP, _I32, _FP,
+                                      _FP, _FP]),
     "mdtw_threshold": (C.c_int, [_P, _P, _P, _I64, _F, C.POINTER(_I64)]),
     "mdtw_argmin": (C.c_int, [_P, _P, _I64, C.POINTER(_I64), _FP]),
 }

@@ -384,3 +384,30 @@ class Batch:
             pos.ctypes.data_as(C.POINTER(C.c_int64)), _fp(dist), infos, C.byref(ms)),
             "mdtw_batch_search")
         return BatchResult(pos, dist, [i.as_dict() for i in infos], ms.value)
+
+    def backward(self, series, positions, grads, metric: int = METRIC_L2,
+                 sqrt_dist: bool = True, raw_queries=None):
+        """Query gradients of sum_j grads[i, j] * dist(i, positions[i, j]) for
+        every query (mdtw_batch_backward).  positions/grads: (nq, k) arrays
+        (position < 0 skipped).  raw_queries: the raw queries (list), giving
+        gradients with respect to them; otherwise with respect to the
+        z-normalised queries.  Returns (list of nq gradient arrays, dist
+        (nq, k))."""
+        eng = self.engine
+        pos = np.ascontiguousarray(positions, dtype=np.int64)
+        k = pos.shape[1]
+        g = np.ascontiguousarray(grads, dtype=np.float32).reshape(pos.shape)
+        out = np.empty(int(self.lengths.sum()), np.float32)
+        dist = np.empty(pos.shape, np.float32)
+        raw = None
+        if raw_queries is not None:
+            raw = np.ascontiguousarray(np.concatenate(
+                [np.asarray(q, np.float32) for q in raw_queries]), dtype=np.float32)
+        eng._stream()
+        check(eng.lib.mdtw_batch_backward(
+            self.h, eng._f32(series, "series"), series.numel(), int(metric), int(k),
+            pos.ctypes.data_as(C.POINTER(C.c_int64)), _fp(g), 1 if sqrt_dist else 0,
+            _fp(raw) if raw is not None else None, _fp(out), _fp(dist)),
+            "mdtw_batch_backward")
+        splits = np.cumsum(self.lengths)[:-1]
+        return np.split(out, splits), dist

@@ -37,8 +37,8 @@ struct BwdScalars {
     int32_t m;
     int32_t w;
     int32_t sqrt_dist;
-    int32_t unused;
-    int64_t words;  // direction words per window
+    int32_t pair_stats;  // 1: avg/sd hold one entry per window of the list
+    int64_t words;       // direction words per window
 };
 
 template <int C, bool L2>
@@ -54,8 +54,8 @@ __global__ __launch_bounds__(64) void bwd_path_kernel(
     const int64_t pr = blockIdx.x;
     const int64_t pos = idx[pr];
     const float* x = s + pos;
-    const float mu = avg[pos];
-    const float sg = sd[pos];
+    const float mu = a.pair_stats ? avg[pr] : avg[pos];
+    const float sg = a.pair_stats ? sd[pr] : sd[pos];
     float* lq = lds;
     float* lz = lds + m;
     for (int t = lane; t < m; t += kWave) {
@@ -268,6 +268,55 @@ PathFn pick_path(int C, bool l2) {
 }
 
 }  // namespace
+
+size_t bwd_ws_bytes(int32_t m, int32_t w, int64_t count) {
+    const int C = slots_per_lane(w);
+    const int64_t groups = (2 * static_cast<int64_t>(m) - 1 + kDirGroup - 1) / kDirGroup;
+    const int64_t words = groups * kWave * C;
+    const size_t per_pair = sizeof(uint32_t) * words + sizeof(float) * m;
+    const int64_t chunk = std::max<int64_t>(
+        1, std::min<int64_t>(std::max<int64_t>(count, 1),
+                             static_cast<int64_t>(kBwdChunkBytes / per_pair)));
+    return carve_size({sizeof(double) * m, sizeof(uint32_t) * words * chunk,
+                       sizeof(float) * m * chunk});
+}
+
+int enqueue_backward(hipStream_t st, const float* s, const float* avg, const float* sd,
+                     bool pair_stats, const int32_t* idx, const float* grad, int64_t count,
+                     const float* q, int32_t m, int32_t w, int32_t metric, int32_t sqrt_dist,
+                     const float* q_raw, float* dist, float* out, char* ws) {
+    const int C = slots_per_lane(w);
+    const int64_t groups = (2 * static_cast<int64_t>(m) - 1 + kDirGroup - 1) / kDirGroup;
+    const int64_t words = groups * kWave * C;
+    const size_t per_pair = sizeof(uint32_t) * words + sizeof(float) * m;
+    const int64_t chunk = std::max<int64_t>(
+        1, std::min<int64_t>(std::max<int64_t>(count, 1),
+                             static_cast<int64_t>(kBwdChunkBytes / per_pair)));
+    Carve cv{ws};
+    double* gsum = cv.take<double>(m);
+    uint32_t* dirs = cv.take<uint32_t>(words * chunk);
+    float* gpair = cv.take<float>(static_cast<size_t>(m) * chunk);
+
+    const unsigned mblocks = static_cast<unsigned>((m + 255) / 256);
+    hipLaunchKernelGGL(bwd_fill_kernel, dim3(mblocks), dim3(256), 0, st, gsum, m);
+    int rc = hip_status(hipGetLastError());
+    if (rc != MDTW_OK) return rc;
+    const PathFn fn = pick_path(C, metric == MDTW_METRIC_L2);
+    const BwdScalars sc{m, w, sqrt_dist ? 1 : 0, pair_stats ? 1 : 0, words};
+    const size_t lds = sizeof(float) * 2 * static_cast<size_t>(m);
+    for (int64_t off = 0; off < count; off += chunk) {
+        const int64_t n = std::min(chunk, count - off);
+        hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(n)), dim3(kWave), lds, st, s,
+                           pair_stats ? avg + off : avg, pair_stats ? sd + off : sd, idx + off, q,
+                           grad + off, dirs, gpair, dist ? dist + off : nullptr, sc);
+        hipLaunchKernelGGL(bwd_accum_kernel, dim3(mblocks), dim3(256), 0, st, gpair, n, m, gsum);
+        rc = hip_status(hipGetLastError());
+        if (rc != MDTW_OK) return rc;
+    }
+    hipLaunchKernelGGL(bwd_final_kernel, dim3(1), dim3(kFinThreads), 0, st, gsum, q_raw, m, out);
+    return hip_status(hipGetLastError());
+}
+
 }  // namespace mdtw
 
 using namespace mdtw;
@@ -288,44 +337,10 @@ extern "C" int mdtw_cdtw_backward(mdtw_engine* e, const float* d_series, const f
     if (wc > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
     if (count > 0x7fffffffll) return MDTW_ERR_UNSUPPORTED;
     DeviceGuard guard(e->device);
-
-    const int C = slots_per_lane(wc);
-    const int64_t groups = (2 * static_cast<int64_t>(m) - 1 + kDirGroup - 1) / kDirGroup;
-    const int64_t words = groups * kWave * C;
-    const size_t per_pair = sizeof(uint32_t) * words + sizeof(float) * m;
-    const int64_t chunk = std::max<int64_t>(
-        1, std::min<int64_t>(std::max<int64_t>(count, 1),
-                             static_cast<int64_t>(kBwdChunkBytes / per_pair)));
-    const size_t bytes = carve_size({sizeof(double) * m,
-                                     sizeof(uint32_t) * words * chunk,
-                                     sizeof(float) * m * chunk});
     void* ws = nullptr;
-    int rc = ws_reserve(e, bytes, &ws);
+    int rc = ws_reserve(e, bwd_ws_bytes(m, wc, count), &ws);
     if (rc != MDTW_OK) return rc;
-    Carve cv{static_cast<char*>(ws)};
-    double* gsum = cv.take<double>(m);
-    uint32_t* dirs = cv.take<uint32_t>(words * chunk);
-    float* gpair = cv.take<float>(static_cast<size_t>(m) * chunk);
-
-    hipStream_t st = e->stream;
-    const unsigned mblocks = static_cast<unsigned>((m + 255) / 256);
-    hipLaunchKernelGGL(bwd_fill_kernel, dim3(mblocks), dim3(256), 0, st, gsum, m);
-    rc = hip_status(hipGetLastError());
-    if (rc != MDTW_OK) return rc;
-
-    const PathFn fn = pick_path(C, metric == MDTW_METRIC_L2);
-    const BwdScalars sc{m, wc, sqrt_dist ? 1 : 0, 0, words};
-    const size_t lds = sizeof(float) * 2 * static_cast<size_t>(m);
-    for (int64_t off = 0; off < count; off += chunk) {
-        const int64_t n = std::min(chunk, count - off);
-        hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(n)), dim3(kWave), lds, st, d_series,
-                           d_avg, d_std, d_indices + off, e->d_q, d_grad + off, dirs, gpair,
-                           d_dist ? d_dist + off : nullptr, sc);
-        hipLaunchKernelGGL(bwd_accum_kernel, dim3(mblocks), dim3(256), 0, st, gpair, n, m, gsum);
-        rc = hip_status(hipGetLastError());
-        if (rc != MDTW_OK) return rc;
-    }
-    hipLaunchKernelGGL(bwd_final_kernel, dim3(1), dim3(kFinThreads), 0, st, gsum, d_query_raw, m,
-                       d_grad_query);
-    return hip_status(hipGetLastError());
+    return enqueue_backward(e->stream, d_series, d_avg, d_std, false, d_indices, d_grad, count,
+                            e->d_q, m, wc, metric, sqrt_dist, d_query_raw, d_dist, d_grad_query,
+                            static_cast<char*>(ws));
 }

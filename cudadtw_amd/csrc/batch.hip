@@ -89,6 +89,13 @@ struct mdtw_batch {
     std::map<int, LaunchShape> shapes;
     uint64_t shapes_version = 0;
 
+    // Backward: one device arena for the pair lists, statistics, raw
+    // queries and outputs, and one scratch per pipeline (grown on demand).
+    char* bwd_buf = nullptr;
+    size_t bwd_cap = 0;
+    std::vector<char*> bwd_ws;
+    size_t bwd_ws_cap = 0;
+
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -121,6 +128,9 @@ struct mdtw_batch {
         drop_graph();
         free_outputs();
         free_ws();
+        if (bwd_buf) (void)hipFree(bwd_buf);
+        for (char* p : bwd_ws)
+            if (p) (void)hipFree(p);
         if (d_arena) (void)hipFree(d_arena);
         for (hipStream_t s : streams)
             if (s) (void)hipStreamDestroy(s);
@@ -575,6 +585,131 @@ int mdtw_batch_search(mdtw_batch* b, const float* d_series, int64_t n, int32_t m
                 inf.threshold = d;
             }
             inf.elapsed_ms = ms;
+        }
+    }
+    return MDTW_OK;
+}
+
+int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n, int32_t metric,
+                        int32_t k, const int64_t* h_pos, const float* h_grad,
+                        int32_t sqrt_dist, const float* h_query_raw, float* h_grad_out,
+                        float* h_dist) {
+    int rc = check_common(b, d_series, n, metric, k);
+    if (rc != MDTW_OK) return rc;
+    if (!h_pos || !h_grad || !h_grad_out) return MDTW_ERR_INVALID_ARGUMENT;
+    mdtw_engine* e = b->e;
+    DeviceGuard guard(e->device);
+    const int32_t nq = b->nq;
+    // Pair lists, query by query (entries with position < 0 are skipped).
+    std::vector<int64_t> po(nq + 1, 0), qo(nq + 1, 0);
+    std::vector<int64_t> pos64;
+    std::vector<int32_t> pos32, plen;
+    std::vector<float> pgrad;
+    for (int32_t i = 0; i < nq; ++i) {
+        for (int32_t j = 0; j < k; ++j) {
+            const int64_t p = h_pos[static_cast<size_t>(i) * k + j];
+            if (p < 0) continue;
+            if (p > n - b->m[i]) return MDTW_ERR_INVALID_ARGUMENT;
+            pos64.push_back(p);
+            pos32.push_back(static_cast<int32_t>(p));
+            plen.push_back(b->m[i]);
+            pgrad.push_back(h_grad[static_cast<size_t>(i) * k + j]);
+        }
+        po[i + 1] = static_cast<int64_t>(pos64.size());
+        qo[i + 1] = qo[i] + b->m[i];
+    }
+    const int64_t np = po[nq], nqf = qo[nq];
+    const size_t bytes = carve_size({sizeof(int64_t) * np, sizeof(int32_t) * np,
+                                     sizeof(int32_t) * np, sizeof(float) * np,
+                                     sizeof(float) * np, sizeof(float) * np, sizeof(float) * np,
+                                     sizeof(float) * nqf, sizeof(float) * nqf});
+    if (bytes > b->bwd_cap) {
+        if (b->bwd_buf) (void)hipFree(b->bwd_buf);
+        b->bwd_buf = nullptr;
+        b->bwd_cap = 0;
+        if (hipMalloc(&b->bwd_buf, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return MDTW_ERR_ALLOC;
+        }
+        b->bwd_cap = bytes;
+    }
+    size_t wsb = 0;
+    for (int32_t i = 0; i < nq; ++i)
+        wsb = std::max(wsb, bwd_ws_bytes(b->m[i], b->w[i], po[i + 1] - po[i]));
+    if (wsb > b->bwd_ws_cap || static_cast<int>(b->bwd_ws.size()) != b->P) {
+        for (char* p : b->bwd_ws)
+            if (p) (void)hipFree(p);
+        b->bwd_ws.clear();
+        b->bwd_ws_cap = 0;
+        for (int p = 0; p < b->P; ++p) {
+            char* ptr = nullptr;
+            if (hipMalloc(&ptr, wsb) != hipSuccess) {
+                (void)hipGetLastError();
+                return MDTW_ERR_ALLOC;
+            }
+            b->bwd_ws.push_back(ptr);
+        }
+        b->bwd_ws_cap = wsb;
+    }
+    Carve cv{b->bwd_buf};
+    int64_t* d_pos64 = cv.take<int64_t>(np);
+    int32_t* d_pos32 = cv.take<int32_t>(np);
+    int32_t* d_len = cv.take<int32_t>(np);
+    float* d_grad = cv.take<float>(np);
+    float* d_mu = cv.take<float>(np);
+    float* d_sd = cv.take<float>(np);
+    float* d_dist = cv.take<float>(np);
+    float* d_raw = cv.take<float>(nqf);
+    float* d_out = cv.take<float>(nqf);
+
+    hipStream_t st = b->origin;
+    rc = hip_ok(hipMemcpyAsync(d_pos64, pos64.data(), sizeof(int64_t) * np,
+                               hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(d_pos32, pos32.data(), sizeof(int32_t) * np,
+                                   hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(d_len, plen.data(), sizeof(int32_t) * np,
+                                   hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(d_grad, pgrad.data(), sizeof(float) * np,
+                                   hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK && h_query_raw)
+        rc = hip_ok(hipMemcpyAsync(d_raw, h_query_raw, sizeof(float) * nqf,
+                                   hipMemcpyHostToDevice, st));
+    // Window statistics of the matched windows, as the forward computed them.
+    if (rc == MDTW_OK) rc = launch_stats_at(st, d_series, n, d_len, d_pos64, np, d_mu, d_sd);
+    if (rc == MDTW_OK) rc = hip_ok(hipEventRecord(b->fork, st));
+    for (int p = 0; p < b->P && rc == MDTW_OK; ++p)
+        rc = hip_ok(hipStreamWaitEvent(b->streams[p], b->fork, 0));
+    for (int32_t i = 0; i < nq && rc == MDTW_OK; ++i) {
+        const int p = i % b->P;
+        rc = enqueue_backward(b->streams[p], d_series, d_mu + po[i], d_sd + po[i], true,
+                              d_pos32 + po[i], d_grad + po[i], po[i + 1] - po[i],
+                              b->d_arena + b->off[i], b->m[i], b->w[i], metric, sqrt_dist,
+                              h_query_raw ? d_raw + qo[i] : nullptr, d_dist + po[i],
+                              d_out + qo[i], b->bwd_ws[p]);
+    }
+    for (int p = 0; p < b->P && rc == MDTW_OK; ++p) {
+        rc = hip_ok(hipEventRecord(b->join[p], b->streams[p]));
+        if (rc == MDTW_OK) rc = hip_ok(hipStreamWaitEvent(st, b->join[p], 0));
+    }
+    std::vector<float> dist(static_cast<size_t>(np));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(h_grad_out, d_out, sizeof(float) * nqf, hipMemcpyDeviceToHost,
+                                   st));
+    if (rc == MDTW_OK && h_dist && np)
+        rc = hip_ok(hipMemcpyAsync(dist.data(), d_dist, sizeof(float) * np,
+                                   hipMemcpyDeviceToHost, st));
+    if (rc == MDTW_OK) rc = hip_ok(hipStreamSynchronize(st));
+    if (rc != MDTW_OK) return rc;
+    if (h_dist) {
+        for (int32_t i = 0; i < nq; ++i) {
+            int64_t q = po[i];
+            for (int32_t j = 0; j < k; ++j) {
+                float& o = h_dist[static_cast<size_t>(i) * k + j];
+                o = h_pos[static_cast<size_t>(i) * k + j] >= 0 ? dist[q++] : INFINITY;
+            }
         }
     }
     return MDTW_OK;

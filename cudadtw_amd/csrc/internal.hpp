@@ -223,12 +223,26 @@ int launch_topk_select(hipStream_t st, const Ctrl* ctrl, const unsigned long lon
                        int32_t k, int64_t excl, int32_t metric, int64_t* out_pos,
                        float* out_dist, int32_t* overflow);
 
+// Query gradient of sum_p grad[p] * dist_p over windows idx[0..count) of the
+// query q (m, band w): recompute, path walk, z-normalisation Jacobian when
+// q_raw is given (backward.hip).  avg/sd: window statistics indexed by window
+// (pair_stats false) or by list entry (true).  ws: bwd_ws_bytes(m, w, count).
+size_t bwd_ws_bytes(int32_t m, int32_t w, int64_t count);
+int enqueue_backward(hipStream_t st, const float* s, const float* avg, const float* sd,
+                     bool pair_stats, const int32_t* idx, const float* grad, int64_t count,
+                     const float* q, int32_t m, int32_t w, int32_t metric, int32_t sqrt_dist,
+                     const float* q_raw, float* dist, float* out, char* ws);
+
 // ---- kernel launchers (each in its own translation unit) -----------------
 // stats.hip
 int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,
                       float* avg, float* sd);
 int launch_stats_refcompat(mdtw_engine* e, const float* s, int64_t n, int32_t m,
                            float* avg, float* sd);
+// MDTW_STATS_FP64 statistics of listed windows (window pos[p], length len[p]),
+// bit-identical to launch_stats_fp64's for the same window.
+int launch_stats_at(hipStream_t st, const float* s, int64_t n, const int32_t* len,
+                    const int64_t* pos, int64_t count, float* avg, float* sd);
 // bounds.hip
 int launch_lb_kim(hipStream_t st, const float* s, const float* avg,
                   const float* sd, const float* q, int32_t m,

@@ -123,17 +123,13 @@ __device__ __forceinline__ void stats_wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
-    const float* __restrict__ s, int64_t L, int32_t m, float* __restrict__ avg,
-    float* __restrict__ sd) {
-    __shared__ float xa_all[kStatsWaves][kStatsLds];
-    __shared__ float xb_all[kStatsWaves][kStatsLds];
+// One tile [w0, w0 + nw) of windows by one wave (w0 a multiple of kStatsTW):
+// leaves mean and sigma of window w0 + i in xa / xb at stats_skew(i).  The
+// whole-stream kernel and the window-list kernel below both run exactly this,
+// so a window's statistics do not depend on which of them computed it.
+__device__ __forceinline__ int stats_tile(const float* __restrict__ s, int64_t L, int32_t m,
+                                          int64_t w0, float* xa, float* xb) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
-    float* xa = xa_all[wid];
-    float* xb = xb_all[wid];
-    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
-    if (w0 >= L) return;
     const int nw = L - w0 < kStatsTW ? static_cast<int>(L - w0) : kStatsTW;
     const int64_t last = L - 1 + m - 1;  // last stream sample
     const float* x = s + w0;
@@ -219,6 +215,21 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
         q2 += xn * xn + xo * xo;
     }
     stats_wave_fence();
+    return nw;
+}
+
+__global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
+    const float* __restrict__ s, int64_t L, int32_t m, float* __restrict__ avg,
+    float* __restrict__ sd) {
+    __shared__ float xa_all[kStatsWaves][kStatsLds];
+    __shared__ float xb_all[kStatsWaves][kStatsLds];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    float* xa = xa_all[wid];
+    float* xb = xb_all[wid];
+    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
+    if (w0 >= L) return;
+    const int nw = stats_tile(s, L, m, w0, xa, xb);
 #pragma unroll 4
     for (int v = 0; v < kStatsR; ++v) {
         const int i = v * kWave + lane;
@@ -226,6 +237,28 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
             __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
             __builtin_nontemporal_store(xb[stats_skew(i)], sd + w0 + i);
         }
+    }
+}
+
+// Statistics of listed windows: entry p is window pos[p] of length len[p]
+// over the stream s[0..n); one wave computes that window's tile.
+__global__ __launch_bounds__(kWave) void stats_at_kernel(const float* __restrict__ s, int64_t n,
+                                                         const int32_t* __restrict__ len,
+                                                         const int64_t* __restrict__ pos,
+                                                         int64_t count, float* __restrict__ avg,
+                                                         float* __restrict__ sd) {
+    __shared__ float xa[kStatsLds];
+    __shared__ float xb[kStatsLds];
+    const int64_t p = blockIdx.x;
+    if (p >= count) return;
+    const int32_t m = len[p];
+    const int64_t w = pos[p];
+    const int64_t L = n - m + 1;
+    const int64_t w0 = (w / kStatsTW) * kStatsTW;
+    stats_tile(s, L, m, w0, xa, xb);
+    if (threadIdx.x == 0) {
+        avg[p] = xa[stats_skew(static_cast<int>(w - w0))];
+        sd[p] = xb[stats_skew(static_cast<int>(w - w0))];
     }
 }
 
@@ -331,6 +364,19 @@ __global__ void ref_finalize(const double* __restrict__ P1, const double* __rest
 }
 
 }  // namespace
+
+int launch_stats_at(hipStream_t st, const float* s, int64_t n, const int32_t* len,
+                    const int64_t* pos, int64_t count, float* avg, float* sd) {
+    if (count <= 0) return MDTW_OK;
+    hipLaunchKernelGGL(stats_at_kernel, dim3(static_cast<unsigned>(count)), dim3(kWave), 0, st,
+                       s, n, len, pos, count, avg, sd);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
 
 int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m, float* avg,
                       float* sd) {

@@ -219,6 +219,20 @@ int mdtw_batch_search(mdtw_batch* b, const float* d_series, int64_t n,
                       int64_t* h_pos, float* h_dist, mdtw_search_info* h_info,
                       float* h_elapsed_ms);
 
+/* No reference counterpart.  Query gradients for the whole batch: for every
+ * query i, the gradient of  sum_j h_grad[i*k + j] * dist(i, h_pos[i*k + j])
+ * (positions < 0 skipped; dist as mdtw_cdtw_backward, sqrt when sqrt_dist,
+ * window statistics computed as the batch search computes them).
+ * h_query_raw: the raw queries concatenated (gradients with respect to them,
+ * through each z-normalisation), or NULL (with respect to the z-normalised
+ * queries).  h_grad_out: the nq gradients concatenated (sum of lengths);
+ * h_dist (nq*k, may be NULL): the recomputed distances (+inf where skipped). */
+int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n,
+                        int32_t metric, int32_t k, const int64_t* h_pos,
+                        const float* h_grad, int32_t sqrt_dist,
+                        const float* h_query_raw, float* h_grad_out,
+                        float* h_dist);
+
 /* ---- primitives (cub_util.cuh) ------------------------------------------- */
 /* cub_util.cuh:154-172 threshold: keeps d_indices[i] with d_values[i] <= thr,
  * in order, in place; the new length goes to *h_count. */

@@ -107,3 +107,41 @@ def test_batch_argument_errors(engine):
     with pytest.raises(MdtwError):
         b.search(dev(synth.random_walk(1000, 2)), k=0)
     b.close()
+
+
+@pytest.mark.parametrize("raw", [False, True])
+def test_batch_backward_equals_per_query(engine, raw):
+    s = synth.ecg_like(1 << 15, 12)
+    ds = dev(s)
+    specs = [(64, 10, "cut"), (100, 20, "walk"), (128, 5, "cut"), (150, 90, "walk"),
+             (40, 10, "cut")]
+    rng = np.random.default_rng(4)
+    raws = []
+    qs, bands = [], []
+    for i, (m, p, kind) in enumerate(specs):
+        q = (synth.cut_query(s, m, 60 + i)[0] if kind == "cut"
+             else synth.random_walk(m, 70 + i)).astype(np.float32)
+        raws.append(q)
+        qs.append(O.znormalize_f32(q))
+        bands.append(band_width(m, p))
+    bands = np.array(bands, np.int32)
+    b = engine.batch(qs, bands, pipelines=3)
+    k = 4
+    r = b.search(ds, k=k)
+    grads = rng.uniform(0.5, 2.0, size=(len(qs), k)).astype(np.float32)
+    pos = r.positions.copy()
+    pos[1, 2] = -1  # a skipped entry
+    gb, db = b.backward(ds, pos, grads, raw_queries=raws if raw else None)
+    for i, (q, w) in enumerate(zip(qs, bands)):
+        engine.set_query(q, int(w))
+        avg, sd = engine.avg_std(ds, q.size)
+        keep = pos[i] >= 0
+        idx = torch.from_numpy(pos[i][keep].astype(np.int32)).cuda()
+        g1, d1 = engine.cdtw_backward(ds, avg, sd, idx, dev(grads[i][keep]), int(w),
+                                      query_raw=dev(raws[i]) if raw else None, want_dist=True)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(gb[i], g1.cpu().numpy(), err_msg=f"query {i}")
+        np.testing.assert_array_equal(db[i][keep], d1.cpu().numpy())
+        assert np.all(np.isinf(db[i][~keep]))
+        np.testing.assert_array_equal(db[i][keep], r.distances[i][keep])
+    b.close()
