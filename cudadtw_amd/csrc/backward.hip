@@ -20,8 +20,15 @@
 // distance is bit-identical to mdtw_cdtw's.  Direction codes (0 = (i-1,j-1),
 // 1 = (i-1,j), 2 = (i,j-1); ties: diagonal if not above either other, then
 // (i-1,j) if not above (i,j-1)) of 16 consecutive diagonals are packed into
-// one 32-bit word per slot and stored to a scratch buffer,
-// word (k/16, p) at ((k/16) * 64C + (p % C) * 64 + p / C): coalesced stores.
+// one 32-bit word per slot, word (k/16, p) at ((k/16) * 64C + (p % C) * 64 +
+// p / C): coalesced stores.  The words live in LDS behind the query and the
+// window when all of them fit in kBwdLdsBytes (about 32 C M + 8 M bytes: M up
+// to 1638 at W < 64, 910 at W < 128, 481 at W < 256), else in a scratch
+// buffer in HBM.  A batch of queries runs every window of the queries that
+// fit in one launch per class C (bwd_path_batch_kernel), so thousands of
+// one-wave walks proceed at once.  The walk back is serial, one
+// dependent word read per path cell, so an LDS read (about 100 cycles) in
+// place of an L2 round trip is what sets its speed.
 #include <algorithm>
 #include <cmath>
 
@@ -32,6 +39,7 @@ namespace {
 
 constexpr int kDirGroup = 16;  // diagonals per direction word (2 bits each)
 constexpr size_t kBwdChunkBytes = size_t(256) << 20;  // scratch per launch
+constexpr size_t kBwdLdsBytes = size_t(64) << 10;     // q, z and direction words in LDS
 
 struct BwdScalars {
     int32_t m;
@@ -41,21 +49,16 @@ struct BwdScalars {
     int64_t words;       // direction words per window
 };
 
-template <int C, bool L2>
-__global__ __launch_bounds__(64) void bwd_path_kernel(
-    const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
-    const int32_t* __restrict__ idx, const float* __restrict__ q, const float* __restrict__ gin,
-    uint32_t* __restrict__ dirs, float* __restrict__ gpair, float* __restrict__ dist,
-    BwdScalars a) {
-    extern __shared__ float lds[];
+// One window: recompute the banded DP, walk the optimal path back and write
+// the window's gradient row go[0..m).  lds holds q and z (2m floats) and,
+// when LDIR, the direction words behind them; otherwise dw (HBM) does.
+template <int C, bool L2, bool LDIR>
+__device__ __forceinline__ void bwd_window(const float* __restrict__ x, float mu, float sg,
+                                           const float* __restrict__ q, int m, int W, float gin,
+                                           int sqrt_dist, uint32_t* __restrict__ dw,
+                                           float* __restrict__ go, float* __restrict__ dist,
+                                           float* lds) {
     const int lane = threadIdx.x;
-    const int m = a.m;
-    const int W = a.w;
-    const int64_t pr = blockIdx.x;
-    const int64_t pos = idx[pr];
-    const float* x = s + pos;
-    const float mu = a.pair_stats ? avg[pr] : avg[pos];
-    const float sg = a.pair_stats ? sd[pr] : sd[pos];
     float* lq = lds;
     float* lz = lds + m;
     for (int t = lane; t < m; t += kWave) {
@@ -64,7 +67,7 @@ __global__ __launch_bounds__(64) void bwd_path_kernel(
     }
     __syncthreads();
 
-    uint32_t* dw = dirs + pr * a.words;
+    if (LDIR) dw = reinterpret_cast<uint32_t*>(lz + m);
     const int P = W + 1;
     const int pfin = W >> 1;
     float pad0[C], pad1[C], A1[C], A2[C];
@@ -139,13 +142,16 @@ __global__ __launch_bounds__(64) void bwd_path_kernel(
     for (int c = 1; c < C; ++c)
         if (c == pfin % C) r = A1[c];
     const float D = __shfl(r, pfin / C, kWave);
-    __threadfence();  // direction words stored above are read by lane 0 below
+    // Direction words stored above are read by lane 0 below.
+    if (LDIR)
+        __syncthreads();
+    else
+        __threadfence();
     if (lane != 0) return;
 
-    float scale = gin[pr];
-    if (a.sqrt_dist) scale = D > 0.0f ? scale * (0.5f / sqrtf(D)) : 0.0f;
-    if (dist) dist[pr] = a.sqrt_dist ? sqrtf(D) : D;
-    float* go = gpair + pr * m;
+    float scale = gin;
+    if (sqrt_dist) scale = D > 0.0f ? scale * (0.5f / sqrtf(D)) : 0.0f;
+    if (dist) *dist = sqrt_dist ? sqrtf(D) : D;
     int i = m - 1, j = m - 1;
     float acc = 0.0f;
     // The path has at most 2M - 1 cells; every step lowers i + j.
@@ -156,9 +162,10 @@ __global__ __launch_bounds__(64) void bwd_path_kernel(
         const int k = i + j;
         const int par = (k + W) & 1;
         const int p = min(max((i - j + W - par) >> 1, 0), W);
-        const uint32_t word = __hip_atomic_load(
-            dw + static_cast<int64_t>(k / kDirGroup) * (kWave * C) + (p % C) * kWave + p / C,
-            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int64_t at =
+            static_cast<int64_t>(k / kDirGroup) * (kWave * C) + (p % C) * kWave + p / C;
+        const uint32_t word =
+            LDIR ? dw[at] : __hip_atomic_load(dw + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         int dr = static_cast<int>((word >> (2 * (k & (kDirGroup - 1)))) & 3u);
         if (i == 0) dr = 2;
         else if (j == 0) dr = 1;
@@ -170,6 +177,40 @@ __global__ __launch_bounds__(64) void bwd_path_kernel(
         if (dr != 1) --j;
     }
     go[0] = acc * scale;
+}
+
+template <int C, bool L2, bool LDIR>
+__global__ __launch_bounds__(64) void bwd_path_kernel(
+    const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
+    const int32_t* __restrict__ idx, const float* __restrict__ q, const float* __restrict__ gin,
+    uint32_t* __restrict__ dirs, float* __restrict__ gpair, float* __restrict__ dist,
+    BwdScalars a) {
+    extern __shared__ float lds[];
+    const int64_t pr = blockIdx.x;
+    const int64_t pos = idx[pr];
+    const float mu = a.pair_stats ? avg[pr] : avg[pos];
+    const float sg = a.pair_stats ? sd[pr] : sd[pos];
+    bwd_window<C, L2, LDIR>(s + pos, mu, sg, q, a.m, a.w, gin[pr], a.sqrt_dist,
+                            LDIR ? nullptr : dirs + pr * a.words, gpair + pr * a.m,
+                            dist ? dist + pr : nullptr, lds);
+}
+
+// Every window of a batch of queries, one block each: block b takes pair
+// list[b] of query pair_query[list[b]]; the pair's statistics are mu/sd[pr].
+// All listed queries keep their direction words in LDS.
+template <int C, bool L2>
+__global__ __launch_bounds__(64) void bwd_path_batch_kernel(
+    const float* __restrict__ s, const float* __restrict__ mu, const float* __restrict__ sd,
+    const int32_t* __restrict__ pos, const int32_t* __restrict__ pair_query,
+    const int32_t* __restrict__ list, const BwdQuery* __restrict__ tab,
+    const float* __restrict__ qarena, const float* __restrict__ gin, float* __restrict__ gpair,
+    float* __restrict__ dist, int32_t sqrt_dist) {
+    extern __shared__ float lds[];
+    const int64_t pr = list[blockIdx.x];
+    const BwdQuery Q = tab[pair_query[pr]];
+    bwd_window<C, L2, true>(s + pos[pr], mu[pr], sd[pr], qarena + Q.q_off, Q.m, Q.w, gin[pr],
+                            sqrt_dist, nullptr, gpair + Q.row_begin + (pr - Q.pair_begin) * Q.m,
+                            dist + pr, lds);
 }
 
 __global__ __launch_bounds__(256) void bwd_fill_kernel(double* g, int32_t m) {
@@ -204,10 +245,9 @@ __device__ double block_sum(double v, double* red) {
 }
 
 // out = G (q_raw NULL) or the z-normalisation Jacobian applied to G.
-__global__ __launch_bounds__(kFinThreads) void bwd_final_kernel(const double* __restrict__ gsum,
-                                                                const float* __restrict__ q_raw,
-                                                                int32_t m, float* __restrict__ out) {
-    __shared__ double red[kFinThreads];
+__device__ __forceinline__ void bwd_final(const double* __restrict__ gsum,
+                                          const float* __restrict__ q_raw, int32_t m,
+                                          float* __restrict__ out, double* red) {
     const int t = threadIdx.x;
     if (!q_raw) {
         for (int i = t; i < m; i += kFinThreads) out[i] = static_cast<float>(gsum[i]);
@@ -240,12 +280,43 @@ __global__ __launch_bounds__(kFinThreads) void bwd_final_kernel(const double* __
     }
 }
 
+__global__ __launch_bounds__(kFinThreads) void bwd_final_kernel(const double* __restrict__ gsum,
+                                                                const float* __restrict__ q_raw,
+                                                                int32_t m, float* __restrict__ out) {
+    __shared__ double red[kFinThreads];
+    bwd_final(gsum, q_raw, m, out, red);
+}
+
+// Per query of a batch (blockIdx.x): gsum = sum of its rows, pairs ascending
+// (the order of bwd_accum_kernel, so the sums are the per-query ones).
+__global__ __launch_bounds__(256) void bwd_accum_batch_kernel(const float* __restrict__ gpair,
+                                                              const BwdQuery* __restrict__ tab,
+                                                              double* __restrict__ gsum) {
+    const BwdQuery Q = tab[blockIdx.x];
+    const int i = blockIdx.y * blockDim.x + threadIdx.x;
+    if (!Q.batched || i >= Q.m) return;
+    const float* row = gpair + Q.row_begin + i;
+    double acc = 0.0;
+    for (int64_t p = 0; p < Q.pairs; ++p) acc += static_cast<double>(row[p * Q.m]);
+    gsum[Q.r_off + i] = acc;
+}
+
+__global__ __launch_bounds__(kFinThreads) void bwd_final_batch_kernel(
+    const double* __restrict__ gsum, const float* __restrict__ q_raw,
+    const BwdQuery* __restrict__ tab, float* __restrict__ out) {
+    __shared__ double red[kFinThreads];
+    const BwdQuery Q = tab[blockIdx.x];
+    if (!Q.batched) return;
+    bwd_final(gsum + Q.r_off, q_raw ? q_raw + Q.r_off : nullptr, Q.m, out + Q.r_off, red);
+}
+
 using PathFn = void (*)(const float*, const float*, const float*, const int32_t*, const float*,
                         const float*, uint32_t*, float*, float*, BwdScalars);
 
 template <int C>
-PathFn path_fn(bool l2) {
-    return l2 ? bwd_path_kernel<C, true> : bwd_path_kernel<C, false>;
+PathFn path_fn(bool l2, bool ldir) {
+    if (ldir) return l2 ? bwd_path_kernel<C, true, true> : bwd_path_kernel<C, false, true>;
+    return l2 ? bwd_path_kernel<C, true, false> : bwd_path_kernel<C, false, false>;
 }
 
 int slots_per_lane(int32_t w) {
@@ -257,14 +328,19 @@ int slots_per_lane(int32_t w) {
     return 16;
 }
 
-PathFn pick_path(int C, bool l2) {
+PathFn pick_path(int C, bool l2, bool ldir) {
     switch (C) {
-        case 1: return path_fn<1>(l2);
-        case 2: return path_fn<2>(l2);
-        case 4: return path_fn<4>(l2);
-        case 8: return path_fn<8>(l2);
-        default: return path_fn<16>(l2);
+        case 1: return path_fn<1>(l2, ldir);
+        case 2: return path_fn<2>(l2, ldir);
+        case 4: return path_fn<4>(l2, ldir);
+        case 8: return path_fn<8>(l2, ldir);
+        default: return path_fn<16>(l2, ldir);
     }
+}
+
+// LDS bytes of the path kernel with the direction words in LDS.
+size_t lds_dir_bytes(int32_t m, int64_t words) {
+    return sizeof(float) * 2 * static_cast<size_t>(m) + sizeof(uint32_t) * words;
 }
 
 }  // namespace
@@ -273,11 +349,12 @@ size_t bwd_ws_bytes(int32_t m, int32_t w, int64_t count) {
     const int C = slots_per_lane(w);
     const int64_t groups = (2 * static_cast<int64_t>(m) - 1 + kDirGroup - 1) / kDirGroup;
     const int64_t words = groups * kWave * C;
-    const size_t per_pair = sizeof(uint32_t) * words + sizeof(float) * m;
+    const int64_t gwords = lds_dir_bytes(m, words) <= kBwdLdsBytes ? 0 : words;
+    const size_t per_pair = sizeof(uint32_t) * gwords + sizeof(float) * m;
     const int64_t chunk = std::max<int64_t>(
         1, std::min<int64_t>(std::max<int64_t>(count, 1),
                              static_cast<int64_t>(kBwdChunkBytes / per_pair)));
-    return carve_size({sizeof(double) * m, sizeof(uint32_t) * words * chunk,
+    return carve_size({sizeof(double) * m, sizeof(uint32_t) * gwords * chunk,
                        sizeof(float) * m * chunk});
 }
 
@@ -288,22 +365,24 @@ int enqueue_backward(hipStream_t st, const float* s, const float* avg, const flo
     const int C = slots_per_lane(w);
     const int64_t groups = (2 * static_cast<int64_t>(m) - 1 + kDirGroup - 1) / kDirGroup;
     const int64_t words = groups * kWave * C;
-    const size_t per_pair = sizeof(uint32_t) * words + sizeof(float) * m;
+    const bool ldir = lds_dir_bytes(m, words) <= kBwdLdsBytes;
+    const int64_t gwords = ldir ? 0 : words;
+    const size_t per_pair = sizeof(uint32_t) * gwords + sizeof(float) * m;
     const int64_t chunk = std::max<int64_t>(
         1, std::min<int64_t>(std::max<int64_t>(count, 1),
                              static_cast<int64_t>(kBwdChunkBytes / per_pair)));
     Carve cv{ws};
     double* gsum = cv.take<double>(m);
-    uint32_t* dirs = cv.take<uint32_t>(words * chunk);
+    uint32_t* dirs = cv.take<uint32_t>(gwords * chunk);
     float* gpair = cv.take<float>(static_cast<size_t>(m) * chunk);
 
     const unsigned mblocks = static_cast<unsigned>((m + 255) / 256);
     hipLaunchKernelGGL(bwd_fill_kernel, dim3(mblocks), dim3(256), 0, st, gsum, m);
     int rc = hip_status(hipGetLastError());
     if (rc != MDTW_OK) return rc;
-    const PathFn fn = pick_path(C, metric == MDTW_METRIC_L2);
+    const PathFn fn = pick_path(C, metric == MDTW_METRIC_L2, ldir);
     const BwdScalars sc{m, w, sqrt_dist ? 1 : 0, pair_stats ? 1 : 0, words};
-    const size_t lds = sizeof(float) * 2 * static_cast<size_t>(m);
+    const size_t lds = ldir ? lds_dir_bytes(m, words) : sizeof(float) * 2 * static_cast<size_t>(m);
     for (int64_t off = 0; off < count; off += chunk) {
         const int64_t n = std::min(chunk, count - off);
         hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(n)), dim3(kWave), lds, st, s,
@@ -314,6 +393,64 @@ int enqueue_backward(hipStream_t st, const float* s, const float* avg, const flo
         if (rc != MDTW_OK) return rc;
     }
     hipLaunchKernelGGL(bwd_final_kernel, dim3(1), dim3(kFinThreads), 0, st, gsum, q_raw, m, out);
+    return hip_status(hipGetLastError());
+}
+
+bool bwd_batchable(int32_t m, int32_t w) {
+    const int C = slots_per_lane(w);
+    const int64_t groups = (2 * static_cast<int64_t>(m) - 1 + kDirGroup - 1) / kDirGroup;
+    return lds_dir_bytes(m, groups * kWave * C) <= kBwdLdsBytes;
+}
+
+int bwd_class(int32_t w) { return slots_per_lane(w); }
+
+namespace {
+
+using BatchFn = void (*)(const float*, const float*, const float*, const int32_t*,
+                         const int32_t*, const int32_t*, const BwdQuery*, const float*,
+                         const float*, float*, float*, int32_t);
+
+template <int C>
+BatchFn batch_fn(bool l2) {
+    return l2 ? bwd_path_batch_kernel<C, true> : bwd_path_batch_kernel<C, false>;
+}
+
+}  // namespace
+
+int launch_bwd_path_batch(hipStream_t st, int C, bool l2, int32_t max_m, const float* s,
+                          const float* mu, const float* sd, const int32_t* pos,
+                          const int32_t* pair_query, const int32_t* list, int64_t count,
+                          const BwdQuery* tab, const float* qarena, const float* gin,
+                          float* gpair, float* dist, int32_t sqrt_dist) {
+    if (count <= 0) return MDTW_OK;
+    if (count > 0x7fffffffll) return MDTW_ERR_UNSUPPORTED;
+    BatchFn fn = nullptr;
+    switch (C) {
+        case 1: fn = batch_fn<1>(l2); break;
+        case 2: fn = batch_fn<2>(l2); break;
+        case 4: fn = batch_fn<4>(l2); break;
+        case 8: fn = batch_fn<8>(l2); break;
+        case 16: fn = batch_fn<16>(l2); break;
+        default: return MDTW_ERR_INVALID_ARGUMENT;
+    }
+    const int64_t groups = (2 * static_cast<int64_t>(max_m) - 1 + kDirGroup - 1) / kDirGroup;
+    const size_t lds = lds_dir_bytes(max_m, groups * kWave * C);
+    if (lds > kBwdLdsBytes) return MDTW_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(fn, dim3(static_cast<unsigned>(count)), dim3(kWave), lds, st, s, mu, sd,
+                       pos, pair_query, list, tab, qarena, gin, gpair, dist, sqrt_dist);
+    return hip_status(hipGetLastError());
+}
+
+int launch_bwd_reduce_batch(hipStream_t st, const float* gpair, const BwdQuery* tab, int32_t nq,
+                            int32_t max_m, const float* q_raw, double* gsum, float* out) {
+    if (nq <= 0 || max_m <= 0) return MDTW_OK;
+    const unsigned mblocks = static_cast<unsigned>((max_m + 255) / 256);
+    hipLaunchKernelGGL(bwd_accum_batch_kernel, dim3(static_cast<unsigned>(nq), mblocks), dim3(256),
+                       0, st, gpair, tab, gsum);
+    int rc = hip_status(hipGetLastError());
+    if (rc != MDTW_OK) return rc;
+    hipLaunchKernelGGL(bwd_final_batch_kernel, dim3(static_cast<unsigned>(nq)), dim3(kFinThreads),
+                       0, st, gsum, q_raw, tab, out);
     return hip_status(hipGetLastError());
 }
 

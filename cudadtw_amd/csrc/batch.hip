@@ -619,10 +619,60 @@ int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n, int32_t
         qo[i + 1] = qo[i] + b->m[i];
     }
     const int64_t np = po[nq], nqf = qo[nq];
+    // Queries whose direction words fit in LDS run in the batch kernels: one
+    // launch per class (slots per lane) over all their windows, longest
+    // queries first; the others run query by query on the pipelines.
+    std::vector<BwdQuery> tab(static_cast<size_t>(nq));
+    std::vector<int32_t> pair_query(static_cast<size_t>(np));
+    int64_t rows = 0;
+    int32_t max_m = 0;
+    size_t wsb = 0;
+    bool any_single = false;
+    for (int32_t i = 0; i < nq; ++i) {
+        const int32_t m = b->m[i], w = b->w[i];
+        const int64_t cnt = po[i + 1] - po[i];
+        const bool bt = bwd_batchable(m, w);
+        tab[i] = BwdQuery{po[i], bt ? rows : 0, b->off[i], qo[i], m, w,
+                          static_cast<int32_t>(cnt), bt ? 1 : 0};
+        for (int64_t p = po[i]; p < po[i + 1]; ++p) pair_query[p] = i;
+        if (bt) {
+            rows += cnt * m;
+            max_m = std::max(max_m, m);
+        } else {
+            any_single = true;
+            wsb = std::max(wsb, bwd_ws_bytes(m, w, cnt));
+        }
+    }
+    struct Cls {
+        int C;
+        int64_t begin, count;
+        int32_t max_m;
+    };
+    std::vector<int32_t> list;
+    list.reserve(static_cast<size_t>(np));
+    std::vector<Cls> cls;
+    for (int C = 1; C <= 16; C *= 2) {
+        std::vector<int32_t> qs;
+        for (int32_t i = 0; i < nq; ++i)
+            if (tab[i].batched && bwd_class(b->w[i]) == C) qs.push_back(i);
+        std::stable_sort(qs.begin(), qs.end(),
+                         [&](int32_t x, int32_t y) { return b->m[x] > b->m[y]; });
+        const int64_t begin = static_cast<int64_t>(list.size());
+        int32_t cm = 0;
+        for (int32_t i : qs) {
+            for (int64_t p = po[i]; p < po[i + 1]; ++p) list.push_back(static_cast<int32_t>(p));
+            if (po[i + 1] > po[i]) cm = std::max(cm, b->m[i]);
+        }
+        if (static_cast<int64_t>(list.size()) > begin)
+            cls.push_back(Cls{C, begin, static_cast<int64_t>(list.size()) - begin, cm});
+    }
     const size_t bytes = carve_size({sizeof(int64_t) * np, sizeof(int32_t) * np,
                                      sizeof(int32_t) * np, sizeof(float) * np,
                                      sizeof(float) * np, sizeof(float) * np, sizeof(float) * np,
-                                     sizeof(float) * nqf, sizeof(float) * nqf});
+                                     sizeof(float) * nqf, sizeof(float) * nqf,
+                                     sizeof(BwdQuery) * nq, sizeof(int32_t) * np,
+                                     sizeof(int32_t) * np, sizeof(float) * rows,
+                                     sizeof(double) * nqf});
     if (bytes > b->bwd_cap) {
         if (b->bwd_buf) (void)hipFree(b->bwd_buf);
         b->bwd_buf = nullptr;
@@ -633,10 +683,7 @@ int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n, int32_t
         }
         b->bwd_cap = bytes;
     }
-    size_t wsb = 0;
-    for (int32_t i = 0; i < nq; ++i)
-        wsb = std::max(wsb, bwd_ws_bytes(b->m[i], b->w[i], po[i + 1] - po[i]));
-    if (wsb > b->bwd_ws_cap || static_cast<int>(b->bwd_ws.size()) != b->P) {
+    if (any_single && (wsb > b->bwd_ws_cap || static_cast<int>(b->bwd_ws.size()) != b->P)) {
         for (char* p : b->bwd_ws)
             if (p) (void)hipFree(p);
         b->bwd_ws.clear();
@@ -661,6 +708,11 @@ int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n, int32_t
     float* d_dist = cv.take<float>(np);
     float* d_raw = cv.take<float>(nqf);
     float* d_out = cv.take<float>(nqf);
+    BwdQuery* d_tab = cv.take<BwdQuery>(nq);
+    int32_t* d_pq = cv.take<int32_t>(np);
+    int32_t* d_list = cv.take<int32_t>(np);
+    float* d_rows = cv.take<float>(rows);
+    double* d_gsum = cv.take<double>(nqf);
 
     hipStream_t st = b->origin;
     rc = hip_ok(hipMemcpyAsync(d_pos64, pos64.data(), sizeof(int64_t) * np,
@@ -677,22 +729,46 @@ int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n, int32_t
     if (rc == MDTW_OK && h_query_raw)
         rc = hip_ok(hipMemcpyAsync(d_raw, h_query_raw, sizeof(float) * nqf,
                                    hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(d_tab, tab.data(), sizeof(BwdQuery) * nq,
+                                   hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK && np)
+        rc = hip_ok(hipMemcpyAsync(d_pq, pair_query.data(), sizeof(int32_t) * np,
+                                   hipMemcpyHostToDevice, st));
+    if (rc == MDTW_OK && !list.empty())
+        rc = hip_ok(hipMemcpyAsync(d_list, list.data(), sizeof(int32_t) * list.size(),
+                                   hipMemcpyHostToDevice, st));
     // Window statistics of the matched windows, as the forward computed them.
     if (rc == MDTW_OK) rc = launch_stats_at(st, d_series, n, d_len, d_pos64, np, d_mu, d_sd);
-    if (rc == MDTW_OK) rc = hip_ok(hipEventRecord(b->fork, st));
-    for (int p = 0; p < b->P && rc == MDTW_OK; ++p)
-        rc = hip_ok(hipStreamWaitEvent(b->streams[p], b->fork, 0));
-    for (int32_t i = 0; i < nq && rc == MDTW_OK; ++i) {
-        const int p = i % b->P;
-        rc = enqueue_backward(b->streams[p], d_series, d_mu + po[i], d_sd + po[i], true,
-                              d_pos32 + po[i], d_grad + po[i], po[i + 1] - po[i],
-                              b->d_arena + b->off[i], b->m[i], b->w[i], metric, sqrt_dist,
-                              h_query_raw ? d_raw + qo[i] : nullptr, d_dist + po[i],
-                              d_out + qo[i], b->bwd_ws[p]);
+    if (any_single) {
+        if (rc == MDTW_OK) rc = hip_ok(hipEventRecord(b->fork, st));
+        for (int p = 0; p < b->P && rc == MDTW_OK; ++p)
+            rc = hip_ok(hipStreamWaitEvent(b->streams[p], b->fork, 0));
+        int slot = 0;
+        for (int32_t i = 0; i < nq && rc == MDTW_OK; ++i) {
+            if (tab[i].batched) continue;
+            const int p = slot++ % b->P;
+            rc = enqueue_backward(b->streams[p], d_series, d_mu + po[i], d_sd + po[i], true,
+                                  d_pos32 + po[i], d_grad + po[i], po[i + 1] - po[i],
+                                  b->d_arena + b->off[i], b->m[i], b->w[i], metric, sqrt_dist,
+                                  h_query_raw ? d_raw + qo[i] : nullptr, d_dist + po[i],
+                                  d_out + qo[i], b->bwd_ws[p]);
+        }
     }
-    for (int p = 0; p < b->P && rc == MDTW_OK; ++p) {
-        rc = hip_ok(hipEventRecord(b->join[p], b->streams[p]));
-        if (rc == MDTW_OK) rc = hip_ok(hipStreamWaitEvent(st, b->join[p], 0));
+    for (const Cls& c : cls) {
+        if (rc != MDTW_OK) break;
+        rc = launch_bwd_path_batch(st, c.C, metric == MDTW_METRIC_L2, c.max_m, d_series, d_mu,
+                                   d_sd, d_pos32, d_pq, d_list + c.begin, c.count, d_tab,
+                                   b->d_arena, d_grad, d_rows, d_dist, sqrt_dist);
+    }
+    if (rc == MDTW_OK && max_m > 0)
+        rc = launch_bwd_reduce_batch(st, d_rows, d_tab, nq, max_m, h_query_raw ? d_raw : nullptr,
+                                     d_gsum, d_out);
+    if (any_single) {
+        for (int p = 0; p < b->P && rc == MDTW_OK; ++p) {
+            rc = hip_ok(hipEventRecord(b->join[p], b->streams[p]));
+            if (rc == MDTW_OK) rc = hip_ok(hipStreamWaitEvent(st, b->join[p], 0));
+        }
     }
     std::vector<float> dist(static_cast<size_t>(np));
     if (rc == MDTW_OK)

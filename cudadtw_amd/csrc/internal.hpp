@@ -233,6 +233,33 @@ int enqueue_backward(hipStream_t st, const float* s, const float* avg, const flo
                      const float* q, int32_t m, int32_t w, int32_t metric, int32_t sqrt_dist,
                      const float* q_raw, float* dist, float* out, char* ws);
 
+// One query of a batched backward (device table entry).
+struct BwdQuery {
+    int64_t pair_begin;  // its first entry in the pair lists
+    int64_t row_begin;   // its first gradient row in the row buffer (floats)
+    int64_t q_off;       // its z-normalised query in the query arena
+    int64_t r_off;       // its raw query, gradient sum and output (floats)
+    int32_t m, w;
+    int32_t pairs;       // its windows in the pair lists
+    int32_t batched;     // 1: run by the batch kernels, 0: by enqueue_backward
+};
+// Whether a query's windows can run in the batch kernels (their direction
+// words fit in LDS), and the class (slots per lane) that runs them.
+bool bwd_batchable(int32_t m, int32_t w);
+int bwd_class(int32_t w);
+// Windows list[0..count) (pair indices, queries of class C, m <= max_m):
+// pair pr has window pos[pr], statistics mu/sd[pr], upstream gradient
+// gin[pr]; writes dist[pr] and the gradient rows tab[query].row_begin + ...
+int launch_bwd_path_batch(hipStream_t st, int C, bool l2, int32_t max_m, const float* s,
+                          const float* mu, const float* sd, const int32_t* pos,
+                          const int32_t* pair_query, const int32_t* list, int64_t count,
+                          const BwdQuery* tab, const float* qarena, const float* gin,
+                          float* gpair, float* dist, int32_t sqrt_dist);
+// Row sums (FP64, pairs ascending) and the z-normalisation Jacobian of every
+// batched query of tab[0..nq); gsum and out indexed by r_off.
+int launch_bwd_reduce_batch(hipStream_t st, const float* gpair, const BwdQuery* tab, int32_t nq,
+                            int32_t max_m, const float* q_raw, double* gsum, float* out);
+
 // ---- kernel launchers (each in its own translation unit) -----------------
 // stats.hip
 int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,

@@ -113,8 +113,10 @@ def test_batch_argument_errors(engine):
 def test_batch_backward_equals_per_query(engine, raw):
     s = synth.ecg_like(1 << 15, 12)
     ds = dev(s)
+    # (1024, 10): its direction words exceed LDS, so it runs query by query
+    # beside the batch kernels; (150, 90) is the 4-slots-per-lane class.
     specs = [(64, 10, "cut"), (100, 20, "walk"), (128, 5, "cut"), (150, 90, "walk"),
-             (40, 10, "cut")]
+             (40, 10, "cut"), (1024, 10, "walk")]
     rng = np.random.default_rng(4)
     raws = []
     qs, bands = [], []
@@ -131,6 +133,7 @@ def test_batch_backward_equals_per_query(engine, raw):
     grads = rng.uniform(0.5, 2.0, size=(len(qs), k)).astype(np.float32)
     pos = r.positions.copy()
     pos[1, 2] = -1  # a skipped entry
+    pos[4, :] = -1  # a query with no windows
     gb, db = b.backward(ds, pos, grads, raw_queries=raws if raw else None)
     for i, (q, w) in enumerate(zip(qs, bands)):
         engine.set_query(q, int(w))
