@@ -32,6 +32,7 @@ METRIC_L1 = 0
 METRIC_L2 = 1
 STATS_FP64 = 0
 STATS_REFCOMPAT = 1
+STATS_NONE = 2
 ALGO_CDTW = 0
 ALGO_ED = 1
 ALGO_LDTW = 2

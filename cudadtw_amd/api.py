@@ -23,12 +23,12 @@ import numpy as np
 
 from . import _lib
 from ._lib import (ALGO_CDTW, ALGO_ED, ALGO_LDTW, METRIC_L1, METRIC_L2, STATS_FP64,
-                   STATS_REFCOMPAT, check)
+                   STATS_NONE, STATS_REFCOMPAT, check)
 
 __all__ = ["Engine", "SearchResult", "Batch", "BatchResult", "znormalize", "envelope",
            "band_width",
            "ALGO_CDTW", "ALGO_ED", "ALGO_LDTW", "METRIC_L1", "METRIC_L2",
-           "STATS_FP64", "STATS_REFCOMPAT"]
+           "STATS_FP64", "STATS_NONE", "STATS_REFCOMPAT"]
 
 
 def _fp(a: np.ndarray):
@@ -218,13 +218,16 @@ class Engine:
         return out[:n]
 
     def ldtw(self, series, avg, sd, w: int, count=None, metric=METRIC_L2):
-        """ldtw.cuh:294-391 ztsh2_ldtw: greedy LDTW of every window."""
+        """ldtw.cuh:294-391 ztsh2_ldtw: greedy LDTW of every window; with
+        avg = sd = None the raw windows (ldtw.cuh:198-291 tsh2_ldtw)."""
         n = self._count(None, count, series)
         out = self.empty(max(n, 1))
+        raw = avg is None and sd is None
         self._stream()
-        check(self.lib.mdtw_ldtw(self.h, self._f32(series, "series"), self._f32(avg, "avg"),
-                                 self._f32(sd, "std"), self._f32(out, "out"), n, int(w),
-                                 int(metric)), "mdtw_ldtw")
+        check(self.lib.mdtw_ldtw(self.h, self._f32(series, "series"),
+                                 None if raw else self._f32(avg, "avg"),
+                                 None if raw else self._f32(sd, "std"), self._f32(out, "out"), n,
+                                 int(w), int(metric)), "mdtw_ldtw")
         return out[:n]
 
     def ed(self, series, avg, sd, count=None):

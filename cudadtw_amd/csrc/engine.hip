@@ -303,6 +303,7 @@ int mdtw_avg_std(mdtw_engine* e, const float* d_series, int64_t n, int32_t m, fl
     if (mode == MDTW_STATS_FP64) return launch_stats_fp64(e->stream, d_series, n, m, d_avg, d_std);
     if (mode == MDTW_STATS_REFCOMPAT)
         return launch_stats_refcompat(e, d_series, n, m, d_avg, d_std);
+    if (mode == MDTW_STATS_NONE) return launch_stats_none(e->stream, n - m + 1, d_avg, d_std);
     return MDTW_ERR_INVALID_ARGUMENT;
 }
 
@@ -375,10 +376,23 @@ int mdtw_cdtw_wide(mdtw_engine* e, const float* d_series, const float* d_avg, co
 
 int mdtw_ldtw(mdtw_engine* e, const float* d_series, const float* d_avg, const float* d_std,
               float* d_out, int64_t count, int32_t w, int32_t metric) {
-    if (!e || !d_series || !d_avg || !d_std || !d_out || count < 0 || w < 0)
+    if (!e || !d_series || !d_out || count < 0 || w < 0)
         return w < 0 ? MDTW_ERR_BAND : MDTW_ERR_INVALID_ARGUMENT;
+    if ((d_avg == nullptr) != (d_std == nullptr)) return MDTW_ERR_INVALID_ARGUMENT;
     if (!e->has_query) return MDTW_ERR_NO_QUERY;
     DeviceGuard g(e->device);
+    if (!d_avg && count > 0) {  // raw windows: mean 0, sigma 1 in the workspace
+        void* ws = nullptr;
+        int rc = ws_reserve(e, carve_size({sizeof(float) * count, sizeof(float) * count}), &ws);
+        if (rc != MDTW_OK) return rc;
+        Carve cv{static_cast<char*>(ws)};
+        float* avg = cv.take<float>(count);
+        float* sd = cv.take<float>(count);
+        rc = launch_stats_none(e->stream, count, avg, sd);
+        if (rc != MDTW_OK) return rc;
+        d_avg = avg;
+        d_std = sd;
+    }
     BandArgs a;
     a.s = d_series;
     a.avg = d_avg;

@@ -264,6 +264,8 @@ int launch_bwd_reduce_batch(hipStream_t st, const float* gpair, const BwdQuery* 
 // stats.hip
 int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,
                       float* avg, float* sd);
+// MDTW_STATS_NONE: avg[0..count) = 0, sd[0..count) = 1.
+int launch_stats_none(hipStream_t st, int64_t count, float* avg, float* sd);
 int launch_stats_refcompat(mdtw_engine* e, const float* s, int64_t n, int32_t m,
                            float* avg, float* sd);
 // MDTW_STATS_FP64 statistics of listed windows (window pos[p], length len[p]),

@@ -190,9 +190,9 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     if (rc == MDTW_OK && topk)
         rc = launch_fill_u32(st, tscratch, static_cast<int64_t>(tighten_ws_words()), 0u);
     if (rc == MDTW_OK && own_stats) {
-        rc = p.stats_mode == MDTW_STATS_REFCOMPAT
-                 ? launch_stats_refcompat(p.e, p.s, p.n, m, avg, sd)
-                 : launch_stats_fp64(st, p.s, p.n, m, avg, sd);
+        rc = p.stats_mode == MDTW_STATS_REFCOMPAT ? launch_stats_refcompat(p.e, p.s, p.n, m, avg, sd)
+             : p.stats_mode == MDTW_STATS_NONE  ? launch_stats_none(st, p.n - m + 1, avg, sd)
+                                                : launch_stats_fp64(st, p.s, p.n, m, avg, sd);
     }
     if (rc != MDTW_OK) return rc;
 
@@ -274,6 +274,9 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     if (k < 1 || k > MDTW_MAX_K) return MDTW_ERR_K;
     if (metric != MDTW_METRIC_L1 && metric != MDTW_METRIC_L2) return MDTW_ERR_INVALID_ARGUMENT;
     if (algo == MDTW_ALGO_ED && metric != MDTW_METRIC_L2) return MDTW_ERR_UNSUPPORTED;
+    if (stats_mode != MDTW_STATS_FP64 && stats_mode != MDTW_STATS_REFCOMPAT &&
+        stats_mode != MDTW_STATS_NONE)
+        return MDTW_ERR_INVALID_ARGUMENT;
     const int32_t m = e->m;
     if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
     const int64_t L = n - m + 1;

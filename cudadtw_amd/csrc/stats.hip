@@ -19,6 +19,7 @@
 // with P[0] = 0 (cub_util.cuh:29-54), (P[i+M] - P[i]) / M in FP64 stored as
 // FP32 (stats.cuh:30-38), sigma = sqrt(fmaf(-X, X, X2)) or 1 (stats.cuh:40-52,
 // nvcc's default contraction of X2 - X*X).
+#include <algorithm>
 #include <cfloat>
 
 #include "internal.hpp"
@@ -64,6 +65,18 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     for (int i = 0; i < kThreads / kWave; ++i) t += sh[i];
     __syncthreads();
     return t;
+}
+
+// ---- MDTW_STATS_NONE: raw windows ------------------------------------------
+// mean 0 and sigma 1 make (x - mean) / sigma == x exactly in every kernel.
+__global__ __launch_bounds__(256) void stats_none_kernel(int64_t count, float* __restrict__ avg,
+                                                         float* __restrict__ sd) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
+         i += stride) {
+        avg[i] = 0.0f;
+        sd[i] = 1.0f;
+    }
 }
 
 // ---- MDTW_STATS_FP64: wave-local tiles -------------------------------------
@@ -386,6 +399,20 @@ int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m, floa
     const int64_t nb = (L + per_block - 1) / per_block;
     hipLaunchKernelGGL(stats_fp64_kernel, dim3(static_cast<unsigned>(nb)),
                        dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+int launch_stats_none(hipStream_t st, int64_t count, float* avg, float* sd) {
+    if (count <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
+    constexpr int64_t per_block = 256 * 4;
+    const int64_t nb = std::min<int64_t>((count + per_block - 1) / per_block, 65536);
+    hipLaunchKernelGGL(stats_none_kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, st,
+                       count, avg, sd);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

@@ -57,6 +57,11 @@ extern "C" {
 #define MDTW_STATS_FP64      0  /* tile-local FP64 prefix sums, FP64 finalize */
 #define MDTW_STATS_REFCOMPAT 1  /* stats.cuh:85-142: FP64 epoch prefix, FP32
                                    finalize fmaf(-X, X, X2), sigma := 1 if <= 0 */
+#define MDTW_STATS_NONE      2  /* no z-normalisation: mean 0, sigma 1 for every
+                                   window, so windows are compared raw (the
+                                   reference's serial_ldtw / tsh2_ldtw,
+                                   ldtw.cuh:18-71,198-291); the query is used
+                                   as given */
 
 /* search algorithm */
 #define MDTW_ALGO_CDTW 0  /* cdtw.cuh:510-592 prune_cdtw            */
@@ -133,7 +138,9 @@ int mdtw_lb_keogh(mdtw_engine* e, const float* d_series, const float* d_avg,
 int mdtw_cdtw(mdtw_engine* e, const float* d_series, const float* d_avg,
               const float* d_std, float* d_out, const int32_t* d_indices,
               int64_t count, int32_t w, int32_t metric);
-/* ldtw.cuh:294-391 ztsh2_ldtw: greedy LDTW of every window 0..count-1. */
+/* ldtw.cuh:294-391 ztsh2_ldtw: greedy LDTW of every window 0..count-1.
+ * d_avg = d_std = NULL: raw windows, ldtw.cuh:198-291 tsh2_ldtw (the query
+ * as given to mdtw_set_query, no z-normalisation). */
 int mdtw_ldtw(mdtw_engine* e, const float* d_series, const float* d_avg,
               const float* d_std, float* d_out, int64_t count, int32_t w,
               int32_t metric);

@@ -265,6 +265,16 @@ inline void ztsh2_ldtw(const float* S, int M, int N, const float* AvgS, const fl
           "ztsh2_ldtw");
 }
 
+// ldtw.cuh:198-291 tsh2_ldtw<Euclidean><<<...>>>(Q, M, S, N, R, W): the same
+// greedy walk on raw values (the query as given to set_query, no window
+// statistics).
+template <bool Euclidean>
+inline void tsh2_ldtw(const float* S, int M, int N, float* R, int W, void* stream = nullptr) {
+    check(mdtw_ldtw(on_stream(stream), S, nullptr, nullptr, R, static_cast<int64_t>(N - M + 1), W,
+                    Euclidean ? MDTW_METRIC_L2 : MDTW_METRIC_L1),
+          "tsh2_ldtw");
+}
+
 }  // namespace mdtw_compat
 
 // ldtw.cuh:177-192 determine_min (host): strict '<', lowest index wins;

@@ -607,6 +607,53 @@ void or_ldtw(const float* zq, const float* s, const float* avg, const float* sd,
     }
 }
 
+/* ldtw.cuh:18-71 serial_ldtw: the same greedy walk on raw values (no
+ * z-normalisation of the window; the query is taken as given). */
+static float ldtw_raw_one(const float* q, const float* x, int32_t M, int32_t W, int l2) {
+    int i = 0, j = 0;
+    float d = q[0] - x[0];
+    d = l2 ? d * d : (d < 0 ? -d : d);
+    while (i + 1 != M || j + 1 != M) {
+        float dg = LDTW_INFTY, dn = LDTW_INFTY, rt = LDTW_INFTY;
+        const int has_i = i + 1 < M, has_j = j + 1 < M;
+        if (has_i && has_j) {
+            dg = q[i + 1] - x[j + 1];
+            dg = l2 ? dg * dg : (dg < 0 ? -dg : dg);
+        }
+        if (has_i && abs(i + 1 - j) <= W) {
+            dn = q[i + 1] - x[j];
+            dn = l2 ? dn * dn : (dn < 0 ? -dn : dn);
+        }
+        if (has_j && abs(j + 1 - i) <= W) {
+            rt = q[i] - x[j + 1];
+            rt = l2 ? rt * rt : (rt < 0 ? -rt : rt);
+        }
+        /* diagonal only if strictly below both; else down if the diagonal is
+           not below it and it is strictly below right; else right */
+        if (dg < dn && dg < rt) {
+            d += dg;
+            ++i;
+            ++j;
+        } else if (!(dg < dn) && dn < rt) {
+            d += dn;
+            ++i;
+        } else {
+            d += rt;
+            ++j;
+        }
+    }
+    return d;
+}
+
+void or_ldtw_raw(const float* q, const float* s, int32_t m, const int32_t* idx,
+                 int64_t count, int32_t w, int32_t metric, float* out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < count; ++g) {
+        int64_t ix = idx ? idx[g] : g;
+        out[g] = ldtw_raw_one(q, s + ix, m, w, metric);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 void or_ed_f64(const double* zq, const float* s, int64_t n, int32_t m,
                const double* avg64, const double* sd64, double* out) {

@@ -26,6 +26,9 @@
  *   or_ldtw                 path property check (every move is the greedy
  *                           choice under the reference's tie rules, the value
  *                           is the path's cost) and LDTW >= cDTW.
+ *   or_ldtw_raw             W = 0 closed form (FP32 running sum of squared
+ *                           differences); equal to or_ldtw with mean 0 and
+ *                           sigma 1; LDTW >= FP64 cDTW of the raw values.
  *   or_ed_f64               numpy direct sum; UCR_ED_BIN location (golden).
  *   or_topk_greedy          brute-force enumeration on tiny inputs.
  *   or_cdtw_grad_f64        FP64 central differences of or_cdtw_f64 (through
@@ -124,6 +127,10 @@ void or_cdtw_grad(const double* zq, const float* s, const double* avg,
 void or_ldtw(const float* zq, const float* s, const float* avg, const float* sd,
              int32_t m, const int32_t* idx, int64_t count, int32_t w,
              int32_t metric, float* out, uint8_t* moves);
+
+/* ldtw.cuh:18-71 serial_ldtw (no z-normalisation), windows idx[0..count). */
+void or_ldtw_raw(const float* q, const float* s, int32_t m, const int32_t* idx,
+                 int64_t count, int32_t w, int32_t metric, float* out);
 
 /* z-normalised Euclidean distance squared, all in FP64 (the quantity
  * ed.cuh:61-69,149 computes through cuFFT: sqrt(2M + bsf) = ED). */

@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
                                     C.c_int32, C.c_int32, C.c_int32, _D, _D, _D, _D]),
             "or_ldtw": (None, [_F, _F, _F, _F, C.c_int32, _I, C.c_int64, C.c_int32, C.c_int32, _F,
                                _U8]),
+            "or_ldtw_raw": (None, [_F, _F, C.c_int32, _I, C.c_int64, C.c_int32, C.c_int32, _F]),
             "or_ed_f64": (None, [_D, _F, C.c_int64, C.c_int32, _D, _D, _D]),
             "or_argmin": (C.c_int64, [_F, C.c_int64]),
             "or_topk_greedy": (C.c_int32, [_F, C.c_int64, C.c_int32, C.c_int64, _L, _F]),
@@ -195,6 +196,16 @@ def ldtw(zq, s, avg, sd, w, idx=None, count=None, metric=1, want_moves=False):
     lib().or_ldtw(_f(zq), _f(s), _f(avg), _f(sd), zq.size, ip, n, int(w), int(metric), _f(out),
                   moves.ctypes.data_as(_U8) if want_moves else None)
     return (out, moves) if want_moves else out
+
+
+def ldtw_raw(q, s, w, idx=None, count=None, metric=1):
+    """ldtw.cuh:18-71 serial_ldtw over windows of s (no z-normalisation)."""
+    s, q = _c32(s), _c32(q)
+    ia, ip = _idx(idx)
+    n = ia.size if ia is not None else (count if count is not None else s.size - q.size + 1)
+    out = np.empty(n, np.float32)
+    lib().or_ldtw_raw(_f(q), _f(s), q.size, ip, n, int(w), int(metric), _f(out))
+    return out
 
 
 def ed_f64(zq64, s, avg64, sd64):

@@ -30,6 +30,7 @@ void caller(float* Subject, float* AvgS, float* StdS, float* out, int* Indices,
     mdtw_compat::match r = prune_cdtw<EUCLIDEAN>(Subject, AvgS, StdS, M, N, W);
     double bsf = calculate_ed(Subject, StdS, M, N);
     mdtw_compat::ztsh2_ldtw<true>(Subject, M, N, AvgS, StdS, R, W);
+    mdtw_compat::tsh2_ldtw<false>(Subject, M, N, R, W);
     determine_min<true>(R, N - M + 1);
     (void)r; (void)bsf;
 }

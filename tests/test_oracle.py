@@ -298,6 +298,52 @@ def test_ldtw_not_below_cdtw():
         assert np.all(l >= d - 1e-4 * d)
 
 
+def test_ldtw_raw_closed_form_at_w0():
+    # W = 0 leaves only the diagonal move: the FP32 running sum of (q_i - x_i)^2.
+    rng = np.random.default_rng(5)
+    s = (7000 + np.cumsum(rng.standard_normal(400))).astype(np.float32)
+    q = (s[100:140] + rng.normal(0, 0.5, 40)).astype(np.float32)
+    got = O.ldtw_raw(q, s, 0)
+    for st in (0, 57, 100, 360):
+        acc = np.float32(0)
+        for i in range(40):
+            d = np.float32(q[i] - s[st + i])
+            acc = np.float32(acc + np.float32(d * d))
+        assert got[st] == acc
+    l1 = O.ldtw_raw(q, s, 0, metric=0)
+    for st in (3, 100):
+        acc = np.float32(0)
+        for i in range(40):
+            acc = np.float32(acc + np.abs(np.float32(q[i] - s[st + i])))
+        assert l1[st] == acc
+
+
+@pytest.mark.parametrize("metric", [1, 0])
+def test_ldtw_raw_equals_z_form_with_identity_stats(metric):
+    # serial_ldtw and z_serial_ldtw (ldtw.cuh:18-71, 74-127) coincide when every
+    # window has mean 0 and sigma 1: (x - 0) / 1 == x in IEEE arithmetic.
+    rng = np.random.default_rng(8)
+    s = (7000 + np.cumsum(rng.standard_normal(3000)) * 0.3).astype(np.float32)
+    q = (s[1200:1264] + rng.normal(0, 0.2, 64)).astype(np.float32)
+    L = s.size - 64 + 1
+    zeros, ones = np.zeros(L, np.float32), np.ones(L, np.float32)
+    for w in (0, 3, 9, 63):
+        a = O.ldtw_raw(q, s, w, metric=metric)
+        b = O.ldtw(q, s, zeros, ones, w, metric=metric)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_ldtw_raw_not_below_raw_cdtw():
+    rng = np.random.default_rng(9)
+    s = (50 + np.cumsum(rng.standard_normal(2000))).astype(np.float32)
+    q = (s[700:748] + rng.normal(0, 0.3, 48)).astype(np.float32)
+    L = s.size - 48 + 1
+    for w in (2, 7):
+        l = O.ldtw_raw(q, s, w)
+        d = O.cdtw_f64(q.astype(np.float64), s, np.zeros(L), np.ones(L), w)
+        assert np.all(l >= d * (1 - 1e-5))
+
+
 # ---- ED, argmin, top-k -------------------------------------------------------------
 def test_ed_matches_numpy():
     s, q, avg, sd, a64, s64 = setup_windows(n=3000, m=100)
