@@ -92,6 +92,15 @@ SIGNATURES = {
                                     _FP, C.POINTER(SearchInfo), _FP]),
     "mdtw_batch_backward": (C.c_int, [_P, _P, _I64, _I32, _I32, C.POINTER(_I64), _FP, _I32, _FP,
                                       _FP, _FP]),
+    "mdtw_stream_create": (C.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I64, _I64,
+                                     C.POINTER(_P)]),
+    "mdtw_stream_destroy": (C.c_int, [_P]),
+    "mdtw_stream_push": (C.c_int, [_P, _FP, _I64]),
+    "mdtw_stream_finish": (C.c_int, [_P]),
+    "mdtw_stream_result": (C.c_int, [_P, C.POINTER(_I64), _FP, C.POINTER(SearchInfo)]),
+    "mdtw_stream_position": (C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64)]),
+    "mdtw_stream_save": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "mdtw_stream_restore": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(_P)]),
     "mdtw_threshold": (C.c_int, [_P, _P, _P, _I64, _F, C.POINTER(_I64)]),
     "mdtw_argmin": (C.c_int, [_P, _P, _I64, C.POINTER(_I64), _FP]),
 }

@@ -295,6 +295,18 @@ class Engine:
         stream (mdtw_batch_*; no reference counterpart)."""
         return Batch(self, zqueries, bands, pipelines)
 
+    # -- streaming search --------------------------------------------------
+    def stream(self, w: int, algo: int = ALGO_CDTW, metric: int = METRIC_L2, k: int = 1,
+               excl: int = 0, stats_mode: int = STATS_FP64,
+               epoch_windows: int = 0) -> "Stream":
+        """A search over host chunks of a stream (mdtw_stream_*), answering
+        as mdtw_search would over everything pushed."""
+        return Stream(self, w, algo, metric, k, excl, stats_mode, epoch_windows)
+
+    def restore_stream(self, state: bytes) -> "Stream":
+        """A stream continuing from Stream.save() (same query required)."""
+        return Stream(self, state=state)
+
     # -- primitives --------------------------------------------------------
     def threshold(self, values, indices, thr: float) -> int:
         """cub_util.cuh:154-172 threshold: stable in-place compaction of the
@@ -326,6 +338,78 @@ class BatchResult:
     distances: np.ndarray   # float32[nq, k]
     infos: list             # per-query search info (dict)
     elapsed_ms: float       # device time of the whole batch (one graph launch)
+
+
+class Stream:
+    """Streaming search: host chunks of any size are pushed, every full
+    device epoch is searched while the next one is copied, and the answer
+    equals the one-shot search over the concatenated samples (the reference's
+    epoch loops, stats.cuh:115-142 and UCR_DTW_BIN.cpp:579-707)."""
+
+    def __init__(self, engine: Engine, w: int = 0, algo: int = ALGO_CDTW,
+                 metric: int = METRIC_L2, k: int = 1, excl: int = 0,
+                 stats_mode: int = STATS_FP64, epoch_windows: int = 0, state: bytes = None):
+        self.engine = engine
+        h = C.c_void_p()
+        engine._stream()
+        if state is None:
+            check(engine.lib.mdtw_stream_create(engine.h, int(w), int(algo), int(metric),
+                                                int(stats_mode), int(k), int(excl),
+                                                int(epoch_windows), C.byref(h)),
+                  "mdtw_stream_create")
+            self.k = int(k)
+        else:
+            buf = C.create_string_buffer(bytes(state), len(state))
+            check(engine.lib.mdtw_stream_restore(engine.h, buf, len(state), C.byref(h)),
+                  "mdtw_stream_restore")
+            # k sits after the magic and five int32 fields of the saved header
+            self.k = int(np.frombuffer(bytes(state), dtype="<i4", count=1, offset=8 + 20)[0])
+        self.h = h
+
+    def push(self, samples) -> None:
+        x = np.ascontiguousarray(samples, dtype=np.float32)
+        self.engine._stream()
+        check(self.engine.lib.mdtw_stream_push(self.h, _fp(x), x.size), "mdtw_stream_push")
+
+    def finish(self) -> None:
+        self.engine._stream()
+        check(self.engine.lib.mdtw_stream_finish(self.h), "mdtw_stream_finish")
+
+    def result(self) -> SearchResult:
+        pos = np.full(self.k, -1, dtype=np.int64)
+        dist = np.full(self.k, np.inf, dtype=np.float32)
+        info = _lib.SearchInfo()
+        self.engine._stream()
+        check(self.engine.lib.mdtw_stream_result(self.h, pos.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                 _fp(dist), C.byref(info)), "mdtw_stream_result")
+        return SearchResult(pos, dist, info.as_dict())
+
+    def position(self):
+        """(samples pushed, windows searched or in flight)."""
+        a, b = C.c_int64(), C.c_int64()
+        check(self.engine.lib.mdtw_stream_position(self.h, C.byref(a), C.byref(b)),
+              "mdtw_stream_position")
+        return a.value, b.value
+
+    def save(self) -> bytes:
+        size = C.c_size_t()
+        self.engine._stream()
+        check(self.engine.lib.mdtw_stream_save(self.h, None, 0, C.byref(size)), "mdtw_stream_save")
+        buf = C.create_string_buffer(size.value)
+        check(self.engine.lib.mdtw_stream_save(self.h, buf, size.value, C.byref(size)),
+              "mdtw_stream_save")
+        return buf.raw[:size.value]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.engine.lib.mdtw_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

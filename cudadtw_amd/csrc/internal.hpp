@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <utility>
 #include <vector>
 
 #include "mdtw.h"
@@ -201,6 +202,11 @@ struct SearchPlan {
     Ctrl* ctrl = nullptr;
     char* ws = nullptr;            // search_ws_bytes(...) of scratch
     LaunchShape shape;
+    // Bounds carried in from windows outside this stream (a streaming search's
+    // earlier epochs): k = 1 starts from this key instead of none; k > 1 caps
+    // the seed bound.  Both must be distances of windows of the whole search.
+    unsigned long long best_init = 0xffffffffffffffffull;
+    float thr_cap = __builtin_huge_valf();
 };
 struct SearchBufs {
     float* avg = nullptr;
@@ -209,6 +215,10 @@ struct SearchBufs {
     int64_t seeded = 0;
 };
 size_t search_ws_bytes(int64_t L, int32_t k, int64_t excl, bool own_stats);
+// Host greedy top-k with exclusion over (distance, position) pairs sorted
+// ascending: a pair is taken unless |position - earlier pick| < excl.
+int32_t greedy_pick(const std::vector<std::pair<float, int64_t>>& sorted, int32_t k, int64_t excl,
+                    int32_t metric, int64_t* h_pos, float* h_dist);
 int64_t search_seed_count(int64_t L, int32_t k, int64_t excl);
 // Enqueues the whole search on p.st (no host synchronisation, no allocation;
 // safe to capture in a graph when stats_mode is FP64).
@@ -321,9 +331,9 @@ int launch_argmin(hipStream_t st, const float* v, int64_t count, Ctrl* ctrl);
 size_t threshold_ws_ints(int64_t count);
 int launch_threshold(hipStream_t st, const float* v, int32_t* idx, int64_t count,
                      float thr, int32_t* tmp, int32_t** d_total);
-// ctrl->thr = k-th smallest of nseeds values (+inf if fewer than k).
+// ctrl->thr = min(cap, k-th smallest of nseeds values (+inf if fewer than k)).
 int launch_seed_threshold(hipStream_t st, const float* seeds, int64_t nseeds,
-                          int32_t k, Ctrl* ctrl);
+                          int32_t k, Ctrl* ctrl, float cap);
 constexpr int64_t kSeedMaxCount = 16384;
 
 }  // namespace mdtw

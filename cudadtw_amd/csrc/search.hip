@@ -128,6 +128,35 @@ int32_t host_topk_select(std::vector<unsigned long long>& keys, int32_t k, int64
     return found;
 }
 
+}  // namespace
+
+namespace mdtw {
+
+int32_t greedy_pick(const std::vector<std::pair<float, int64_t>>& sorted, int32_t k, int64_t excl,
+                    int32_t metric, int64_t* h_pos, float* h_dist) {
+    int32_t found = 0;
+    for (const auto& c : sorted) {
+        if (found == k) break;
+        bool ok = true;
+        for (int32_t p = 0; p < found; ++p) {
+            const int64_t dd = c.second > h_pos[p] ? c.second - h_pos[p] : h_pos[p] - c.second;
+            if (dd < excl) {
+                ok = false;
+                break;
+            }
+        }
+        if (!ok) continue;
+        h_pos[found] = c.second;
+        h_dist[found] = metric == MDTW_METRIC_L2 ? std::sqrt(c.first) : c.first;
+        found++;
+    }
+    return found;
+}
+
+}  // namespace mdtw
+
+namespace {
+
 // One DP-style launch for the selected algorithm.
 int run_algo(const SearchPlan& p, const BandArgs& a) {
     if (p.algo == MDTW_ALGO_CDTW)
@@ -185,7 +214,8 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     bufs->seeded = sp.count;
 
     hipStream_t st = p.st;
-    int rc = launch_ctrl_reset(st, p.ctrl, kBestNone, bmin, tl.bucket_len);
+    const unsigned long long best0 = p.best_init < kBestNone ? p.best_init : kBestNone;
+    int rc = launch_ctrl_reset(st, p.ctrl, best0, bmin, tl.bucket_len);
     if (rc == MDTW_OK && topk) rc = launch_fill_u32(st, bmin, tl.nb, 0x7f800000u);
     if (rc == MDTW_OK && topk)
         rc = launch_fill_u32(st, tscratch, static_cast<int64_t>(tighten_ws_words()), 0u);
@@ -214,7 +244,8 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     sa.mode = topk ? kModeSeed : kModeBest;
     sa.out = seeds;
     rc = run_algo(p, sa);
-    if (rc == MDTW_OK && topk) rc = launch_seed_threshold(st, seeds, sp.count, k, p.ctrl);
+    if (rc == MDTW_OK && topk)
+        rc = launch_seed_threshold(st, seeds, sp.count, k, p.ctrl, p.thr_cap);
     if (rc != MDTW_OK) return rc;
 
     // 3. rounds: the bound left by earlier rounds (k = 1: the best distance

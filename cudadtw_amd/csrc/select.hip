@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void thr_scatter_kernel(const float* __restric
 constexpr int kSeedMax = 16384;
 
 __global__ __launch_bounds__(1024) void seed_thr_kernel(const float* __restrict__ v, int n, int k,
-                                                        Ctrl* ctrl) {
+                                                        float cap, Ctrl* ctrl) {
     __shared__ float a[kSeedMax];
     int size = 1;
     while (size < n) size <<= 1;
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(1024) void seed_thr_kernel(const float* __restrict_
             __syncthreads();
         }
     }
-    if (threadIdx.x == 0) ctrl->thr = (k <= n) ? a[k - 1] : kInf;
+    if (threadIdx.x == 0) ctrl->thr = fminf((k <= n) ? a[k - 1] : kInf, cap);
 }
 
 inline int check_launch() {
@@ -451,10 +451,10 @@ int launch_threshold(hipStream_t st, const float* v, int32_t* idx, int64_t count
 }
 
 int launch_seed_threshold(hipStream_t st, const float* seeds, int64_t nseeds, int32_t k,
-                          Ctrl* ctrl) {
+                          Ctrl* ctrl, float cap) {
     if (nseeds > kSeedMax) return MDTW_ERR_UNSUPPORTED;
     hipLaunchKernelGGL(seed_thr_kernel, dim3(1), dim3(1024), 0, st, seeds,
-                       static_cast<int>(nseeds), k, ctrl);
+                       static_cast<int>(nseeds), k, cap, ctrl);
     return check_launch();
 }
 

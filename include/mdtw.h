@@ -25,6 +25,7 @@
 #ifndef MDTW_H
 #define MDTW_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -239,6 +240,44 @@ int mdtw_batch_backward(mdtw_batch* b, const float* d_series, int64_t n,
                         const float* h_grad, int32_t sqrt_dist,
                         const float* h_query_raw, float* h_grad_out,
                         float* h_dist);
+
+/* ---- streaming search ---------------------------------------------------- */
+/* A search over a stream that arrives in host chunks, for streams longer than
+ * device memory or without an end.  The reference's counterparts are its
+ * epoch loops: avg_std over 10^7-sample epochs with M-1 samples of overlap
+ * (stats.cuh:115-142) and UCR_DTW_BIN.cpp:579-707, which reads fixed chunks
+ * with M-1 overlap and keeps its best-so-far across them.  The answer after
+ * mdtw_stream_finish is the one-shot mdtw_search answer over everything
+ * pushed, bit for bit; positions are int64 stream positions.  The stream
+ * keeps the engine's query at creation (later mdtw_set_query calls do not
+ * affect it) and runs on the engine's stream; it allocates at creation only
+ * (two epoch buffers, its own search workspace). */
+typedef struct mdtw_stream mdtw_stream;
+/* epoch_windows: windows per device epoch, a multiple of 1024 (0: 2^24).
+ * stats_mode: MDTW_STATS_FP64 or MDTW_STATS_NONE (REFCOMPAT statistics need
+ * the whole stream's prefix: MDTW_ERR_UNSUPPORTED). */
+int mdtw_stream_create(mdtw_engine* e, int32_t w, int32_t algo, int32_t metric,
+                       int32_t stats_mode, int32_t k, int64_t excl,
+                       int64_t epoch_windows, mdtw_stream** out);
+int mdtw_stream_destroy(mdtw_stream* s);
+/* Appends count host samples (any count; the buffer may be reused on
+ * return).  Every completed epoch is searched while the next is copied. */
+int mdtw_stream_push(mdtw_stream* s, const float* h_samples, int64_t count);
+/* End of stream: searches the last partial epoch.  No push after it. */
+int mdtw_stream_finish(mdtw_stream* s);
+/* The answer over the windows searched so far (k entries, -1 / +inf past the
+ * matches found); info->windows counts those windows. */
+int mdtw_stream_result(mdtw_stream* s, int64_t* h_pos, float* h_dist,
+                       mdtw_search_info* info);
+/* Samples pushed so far and windows searched or in flight. */
+int mdtw_stream_position(mdtw_stream* s, int64_t* samples_pushed,
+                         int64_t* windows_done);
+/* Resume state: h_buf NULL sets *size only; else writes *size <= cap bytes. */
+int mdtw_stream_save(mdtw_stream* s, void* h_buf, size_t cap, size_t* size);
+/* A stream continuing from a saved state; the engine must hold the same
+ * query (checked by hash: MDTW_ERR_INVALID_ARGUMENT otherwise). */
+int mdtw_stream_restore(mdtw_engine* e, const void* h_buf, size_t size,
+                        mdtw_stream** out);
 
 /* ---- primitives (cub_util.cuh) ------------------------------------------- */
 /* cub_util.cuh:154-172 threshold: keeps d_indices[i] with d_values[i] <= thr,

@@ -1,0 +1,171 @@
+"""Streaming search (mdtw_stream_*) against the one-shot search.
+
+No reference API counterpart (the reference's epoch loops are internal:
+stats.cuh:115-142, UCR_DTW_BIN.cpp:579-707).  The bar: after finish, the
+streamed answer -- positions and distances -- is bit-identical to
+mdtw_search over the whole stream resident on the device (itself checked
+against the oracle's full profile in test_gpu_search.py), for every
+algorithm, k = 1 and top-k, any chunking of the pushes, matches that
+straddle epoch boundaries, and a save / restore in the middle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cudadtw_amd import (ALGO_CDTW, ALGO_ED, ALGO_LDTW, METRIC_L1, METRIC_L2, STATS_FP64,
+                         STATS_NONE, STATS_REFCOMPAT, synth)
+from cudadtw_amd._lib import MdtwError
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EPOCH = 1 << 16
+
+
+def push_chunks(st, s, seed):
+    rng = np.random.default_rng(seed)
+    i = 0
+    while i < s.size:
+        n = int(rng.choice([1, 17, 999, 4096, 30000, 70001, 150000]))
+        st.push(s[i:i + n])
+        i += n
+
+
+def one_shot(engine, s, w, **kw):
+    return engine.search(torch.from_numpy(s).cuda(), w, **kw)
+
+
+def same(a, b):
+    np.testing.assert_array_equal(a.positions, b.positions)
+    np.testing.assert_array_equal(a.distances, b.distances)
+
+
+@pytest.mark.parametrize("algo,metric,k", [(ALGO_CDTW, METRIC_L2, 1), (ALGO_CDTW, METRIC_L2, 6),
+                                           (ALGO_CDTW, METRIC_L1, 4), (ALGO_ED, METRIC_L2, 1),
+                                           (ALGO_ED, METRIC_L2, 5), (ALGO_LDTW, METRIC_L2, 1),
+                                           (ALGO_LDTW, METRIC_L1, 3)])
+def test_stream_equals_one_shot(engine, algo, metric, k):
+    s = synth.ecg_like(300_000, 21)
+    q, _ = synth.cut_query(s, 128, 22)
+    w = 0 if algo == ALGO_ED else 12
+    engine.set_query(O.znormalize_f32(q), w)
+    excl = 64 if k > 1 else 0
+    want = one_shot(engine, s, w, algo=algo, metric=metric, k=k, excl=excl)
+    st = engine.stream(w, algo=algo, metric=metric, k=k, excl=excl, epoch_windows=EPOCH)
+    push_chunks(st, s, 5 + k)
+    st.finish()
+    got = st.result()
+    same(got, want)
+    assert got.info["windows"] == s.size - 128 + 1
+    assert got.info["found"] == want.info["found"]
+    st.close()
+
+
+def test_matches_across_epoch_boundaries(engine):
+    # exact copies of the query planted so that the best windows start just
+    # before, at and just after epoch boundaries (windows >= EPOCH - 127 use
+    # samples of two device buffers' worth of pushes)
+    rng = np.random.default_rng(3)
+    s = synth.random_walk(5 * EPOCH, 30)
+    q = synth.random_walk(128, 31)
+    for p in (EPOCH - 64, 2 * EPOCH, 3 * EPOCH + 1, 4 * EPOCH - 127):
+        s[p:p + 128] = q * (1.0 + 0.01 * rng.standard_normal()) + rng.standard_normal()
+    zq = O.znormalize_f32(q)
+    engine.set_query(zq, 10)
+    for k in (1, 4):
+        want = one_shot(engine, s, 10, k=k, excl=64)
+        st = engine.stream(10, k=k, excl=64, epoch_windows=EPOCH)
+        push_chunks(st, s, k)
+        st.finish()
+        same(st.result(), want)
+        if k == 4:
+            assert sorted(want.positions.tolist()) == [EPOCH - 64, 2 * EPOCH, 3 * EPOCH + 1,
+                                                       4 * EPOCH - 127]
+        st.close()
+
+
+@pytest.mark.parametrize("k", [1, 5])
+def test_save_and_restore_mid_stream(engine, k):
+    s = synth.ecg_like(400_000, 40)
+    q, _ = synth.cut_query(s, 200, 41)
+    engine.set_query(O.znormalize_f32(q), 20)
+    want = one_shot(engine, s, 20, k=k, excl=100)
+    st = engine.stream(20, k=k, excl=100, epoch_windows=EPOCH)
+    cut = 170_123
+    push_chunks(st, s[:cut], 1)
+    state = st.save()
+    st.close()
+    st2 = engine.restore_stream(state)
+    assert st2.position()[0] == cut
+    push_chunks(st2, s[cut:], 2)
+    st2.finish()
+    same(st2.result(), want)
+    st2.close()
+
+
+def test_raw_values_stream(engine):
+    s = synth.metal_like(250_000, 50, offset=7000.0)
+    q, _ = synth.cut_query(s, 96, 51, noise=0.02)
+    engine.set_query(q, 9)
+    for k in (1, 3):
+        want = one_shot(engine, s, 9, k=k, excl=48, stats_mode=STATS_NONE)
+        st = engine.stream(9, k=k, excl=48, stats_mode=STATS_NONE, epoch_windows=EPOCH)
+        push_chunks(st, s, 9)
+        st.finish()
+        same(st.result(), want)
+        st.close()
+
+
+def test_partial_results_and_edges(engine):
+    s = synth.ecg_like(200_000, 60)
+    q, _ = synth.cut_query(s, 128, 61)
+    engine.set_query(O.znormalize_f32(q), 12)
+    st = engine.stream(12, epoch_windows=EPOCH)
+    st.push(s[:100])
+    r = st.result()
+    assert r.positions[0] == -1 and r.info["windows"] == 0
+    st.push(s[100:EPOCH + 127 + 5000])  # one full epoch
+    r = st.result()
+    assert r.info["windows"] == EPOCH
+    prefix = one_shot(engine, s[:EPOCH + 127], 12)
+    same(r, prefix)
+    st.finish()
+    with pytest.raises(MdtwError):
+        st.push(s[:10])
+    st.close()
+    # too short for one window
+    st = engine.stream(12, epoch_windows=EPOCH)
+    st.push(s[:127])
+    st.finish()
+    assert st.result().positions[0] == -1
+    st.close()
+
+
+def test_stream_argument_errors(engine):
+    q = O.znormalize_f32(synth.random_walk(64, 1))
+    engine.set_query(q, 6)
+    with pytest.raises(MdtwError):
+        engine.stream(6, stats_mode=STATS_REFCOMPAT)
+    with pytest.raises(MdtwError):
+        engine.stream(6, epoch_windows=1000)  # not a multiple of 1024
+    with pytest.raises(MdtwError):
+        engine.stream(6, algo=ALGO_ED, metric=METRIC_L1)
+    st = engine.stream(6, epoch_windows=EPOCH)
+    st.push(synth.random_walk(5000, 2))
+    state = st.save()
+    st.close()
+    engine.set_query(O.znormalize_f32(synth.random_walk(64, 3)), 6)
+    with pytest.raises(MdtwError):
+        engine.restore_stream(state)  # another query
+    with pytest.raises(MdtwError):
+        engine.restore_stream(b"garbage" * 20)
+    # the stream keeps its own query copy: a later set_query changes nothing
+    engine.set_query(q, 6)
+    s = synth.ecg_like(100_000, 4)
+    want = one_shot(engine, s, 6, stats_mode=STATS_FP64)
+    st = engine.stream(6, epoch_windows=EPOCH)
+    engine.set_query(O.znormalize_f32(synth.random_walk(64, 9)), 6)
+    st.push(s)
+    st.finish()
+    same(st.result(), want)
+    st.close()
