@@ -153,6 +153,30 @@ def section_c2(eng, torch, args):
     assert r.location == results[w]["best"][0], "pruned search disagrees with the full profile"
     re_, t_ed = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=5)
     log(f"[c2] ED search: {t_ed*1e3:.2f} ms, location {re_.location}")
+
+    # The same search fed from host memory: one-shot after a host-to-device
+    # copy, and the streaming search (2^24-window epochs, pushes of 2^24
+    # samples, copies overlapped with the search).  PCIe included in both.
+    chunk = 1 << 24
+
+    def copied():
+        return eng.search(torch.from_numpy(stream).to("cuda:0"), w)
+
+    def streamed():
+        sst = eng.stream(w, epoch_windows=chunk)
+        for i in range(0, n, chunk):
+            sst.push(stream[i:i + chunk])
+        sst.finish()
+        res = sst.result()
+        sst.close()
+        return res
+
+    rcp, t_copy = timed(torch, copied, reps=2)
+    rst, t_stream = timed(torch, streamed, reps=2)
+    assert rst.location == r.location and rst.distance == r.distance, "streamed answer differs"
+    assert rcp.location == r.location
+    log(f"[c2] from host memory: copy + search {t_copy*1e3:.1f} ms, streamed "
+        f"{t_stream*1e3:.1f} ms (same answer)")
     out = {
         "main": results[w], "L": L, "n": n, "m": m, "w": w,
         "gcups_by_band": {str(k): round(v["gcups"], 2) for k, v in results.items()},
@@ -165,6 +189,10 @@ def section_c2(eng, torch, args):
                           "lb_keogh_pruned": r.info["lb_keogh_pruned"]},
         "ed_search": {"ms": round(t_ed * 1e3, 3), "windows_per_s": round(L / t_ed, 1),
                       "location": re_.location},
+        "from_host": {"copy_then_search_ms": round(t_copy * 1e3, 2),
+                      "streamed_ms": round(t_stream * 1e3, 2),
+                      "streamed_windows_per_s": round(L / t_stream, 1),
+                      "epoch_windows": chunk, "push_samples": chunk},
     }
     return out, stream, q_raw, zq
 

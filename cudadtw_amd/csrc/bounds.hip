@@ -190,8 +190,13 @@ constexpr int kCascadeWaves = 4;
 constexpr int kRing = 256;
 // Survivors are staged per wave in LDS and appended kCandBuf - 64 or more at
 // a time: one atomicAdd on the shared candidate counter per flush instead of
-// one per 64 windows (same-address atomics serialise in L2).
+// one per 64 windows (same-address atomics serialise in L2, ~15 ns each).
+// The last flush and the prune counters are summed per block first: three
+// atomics per block, not per wave, and every wave gets at least
+// kCascadeMinChunks chunks of 64 windows, so a short range (a streaming
+// epoch) does not pay the atomics of a grid sized for a long one.
 constexpr int kCandBuf = 512;
+constexpr int64_t kCascadeMinChunks = 16;
 
 __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -307,10 +312,33 @@ __global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
             if (nbuf > kCandBuf - kWave) flush();
         }
     }
-    if (nbuf) flush();
+    // Block epilogue: one reservation for every wave's remaining survivors.
+    __shared__ unsigned long long part[kCascadeWaves][3];
+    __shared__ unsigned long long bbase;
+    const int wv = threadIdx.x / kWave;
     if (lane == 0) {
-        if (n_kim) atomicAdd(&ctrl->kim, n_kim);
-        if (n_keogh) atomicAdd(&ctrl->keogh, n_keogh);
+        part[wv][0] = static_cast<unsigned long long>(nbuf);
+        part[wv][1] = n_kim;
+        part[wv][2] = n_keogh;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tc = 0, tk = 0, te = 0;
+        for (int v = 0; v < kCascadeWaves; ++v) {
+            tc += part[v][0];
+            tk += part[v][1];
+            te += part[v][2];
+        }
+        bbase = tc ? atomicAdd(&ctrl->count, tc) : 0ull;
+        if (tk) atomicAdd(&ctrl->kim, tk);
+        if (te) atomicAdd(&ctrl->keogh, te);
+    }
+    __syncthreads();
+    unsigned long long off = bbase;
+    for (int v = 0; v < wv; ++v) off += part[v][0];
+    for (int32_t t = lane; t < nbuf; t += kWave) {
+        cand[off + t] = cbuf[t];
+        if (cand_lb) cand_lb[off + t] = lbuf[t];
     }
 }
 
@@ -358,8 +386,10 @@ int launch_cascade(hipStream_t st, const float* s, const float* avg, const float
                    int64_t L, int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
                    float* cand_lb, int64_t max_blocks) {
     if (L <= 0) return MDTW_OK;
-    // Enough waves to cover the chip several times over; each wave strides.
-    const int64_t waves_needed = (L + kWave - 1) / kWave;
+    // Enough waves to cover the chip several times over, each with at least
+    // kCascadeMinChunks chunks of 64 windows; each wave strides.
+    const int64_t chunks = (L + kWave - 1) / kWave;
+    const int64_t waves_needed = (chunks + kCascadeMinChunks - 1) / kCascadeMinChunks;
     const int64_t nb64 = std::max<int64_t>(
         1, std::min<int64_t>((waves_needed + kCascadeWaves - 1) / kCascadeWaves, max_blocks));
     const unsigned nb = static_cast<unsigned>(nb64);

@@ -48,9 +48,28 @@ constexpr int kBandSlideMax = 54;
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 
+// Next 64 windows from the shared cursor (one-wave workgroups).
 __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane) {
     unsigned long long b = 0;
     if (lane == 0) b = atomicAdd(&ctrl->work, static_cast<unsigned long long>(kWave));
+    return static_cast<int64_t>(__shfl(b, 0, kWave));
+}
+
+// Next 64 windows from the shared cursor.  Once a wave of the block has seen
+// the cursor past the end (it only grows), the block's other waves stop
+// without another same-address atomic: at the end of a launch every wave
+// would otherwise queue one on the cursor at once.
+__device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t count, int* drained) {
+    unsigned long long b = 0;
+    if (lane == 0) {
+        if (__hip_atomic_load(drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            b = static_cast<unsigned long long>(count);
+        } else {
+            b = atomicAdd(&ctrl->work, static_cast<unsigned long long>(kWave));
+            if (b >= static_cast<unsigned long long>(count))
+                __hip_atomic_store(drained, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
     return static_cast<int64_t>(__shfl(b, 0, kWave));
 }
 
@@ -258,10 +277,16 @@ __global__ __launch_bounds__(256) void band_kernel(
     unsigned long long n_dp = 0, n_ab = 0;
     uint32_t* const bmin = ctrl->bmin;
     const long long bucket_len = ctrl->bucket_len;
+    __shared__ int drained;
+    __shared__ unsigned long long part[kBandWaves][2];
+    if (a.persistent) {  // uniform: the flag is read by persistent waves only
+        if (threadIdx.x == 0) drained = 0;
+        __syncthreads();
+    }
     // persistent: waves take 64 windows at a time from an atomic cursor;
     // otherwise a static grid stride (every wave the same amount of work).
     const int64_t step = static_cast<int64_t>(gridDim.x) * kBandWaves * kWave;
-    int64_t base = a.persistent ? band_next(ctrl, lane)
+    int64_t base = a.persistent ? band_next(ctrl, lane, count, &drained)
                                 : (static_cast<int64_t>(blockIdx.x) * kBandWaves + wid) * kWave;
     while (base < count) {
         const int64_t g = base + lane;
@@ -294,11 +319,24 @@ __global__ __launch_bounds__(256) void band_kernel(
         }
         n_dp += __popcll(__ballot(valid));
         n_ab += __popcll(__ballot(valid && !alive));
-        base = a.persistent ? band_next(ctrl, lane) : base + step;
+        base = a.persistent ? band_next(ctrl, lane, count, &drained) : base + step;
     }
-    if (lane == 0 && a.mode != kModeProfile) {
-        if (n_dp) atomicAdd(&ctrl->dp, n_dp);
-        if (n_ab) atomicAdd(&ctrl->abandoned, n_ab);
+    // Counters summed per block: two atomics per block, not per wave.
+    if (a.mode != kModeProfile) {
+        if (lane == 0) {
+            part[wid][0] = n_dp;
+            part[wid][1] = n_ab;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long td = 0, ta = 0;
+            for (int v = 0; v < kBandWaves; ++v) {
+                td += part[v][0];
+                ta += part[v][1];
+            }
+            if (td) atomicAdd(&ctrl->dp, td);
+            if (ta) atomicAdd(&ctrl->abandoned, ta);
+        }
     }
 }
 

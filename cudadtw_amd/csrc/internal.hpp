@@ -207,6 +207,8 @@ struct SearchPlan {
     // the seed bound.  Both must be distances of windows of the whole search.
     unsigned long long best_init = 0xffffffffffffffffull;
     float thr_cap = __builtin_huge_valf();
+    // false: no seeding (the carried bounds above already bound the search).
+    bool seed = true;
 };
 struct SearchBufs {
     float* avg = nullptr;

@@ -237,15 +237,19 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     a.res = res;
 
     // 2. seed
-    BandArgs sa = a;
-    sa.count = sp.count;
-    sa.base = 0;
-    sa.stride = sp.stride;
-    sa.mode = topk ? kModeSeed : kModeBest;
-    sa.out = seeds;
-    rc = run_algo(p, sa);
+    if (p.seed) {
+        BandArgs sa = a;
+        sa.count = sp.count;
+        sa.base = 0;
+        sa.stride = sp.stride;
+        sa.mode = topk ? kModeSeed : kModeBest;
+        sa.out = seeds;
+        rc = run_algo(p, sa);
+    } else {
+        bufs->seeded = 0;
+    }
     if (rc == MDTW_OK && topk)
-        rc = launch_seed_threshold(st, seeds, sp.count, k, p.ctrl, p.thr_cap);
+        rc = launch_seed_threshold(st, seeds, p.seed ? sp.count : 0, k, p.ctrl, p.thr_cap);
     if (rc != MDTW_OK) return rc;
 
     // 3. rounds: the bound left by earlier rounds (k = 1: the best distance

@@ -360,6 +360,8 @@ int run_epoch(mdtw_stream* s, int64_t nsamples) {
         p.best_init = static_cast<unsigned long long>(bits) << 32;  // position 0
     }
     if (s->k > 1) p.thr_cap = s->thr;
+    // A bound carried from earlier epochs replaces the seeds' bound.
+    p.seed = s->k == 1 ? s->best_pos < 0 : !(s->thr < INFINITY);
     rc = hip_ok(hipEventRecord(s->start, st));
     if (rc == MDTW_OK) rc = enqueue_search(p, &s->bufs);
     if (rc == MDTW_OK)
