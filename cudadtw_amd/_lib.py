@@ -36,6 +36,7 @@ STATS_NONE = 2
 ALGO_CDTW = 0
 ALGO_ED = 1
 ALGO_LDTW = 2
+ALGO_ED_FFT = 3
 
 
 class SearchInfo(C.Structure):
@@ -80,6 +81,7 @@ SIGNATURES = {
     "mdtw_cdtw_wide": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, _I32, _I32]),
     "mdtw_ldtw": (C.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _I32]),
     "mdtw_ed": (C.c_int, [_P, _P, _P, _P, _P, _I64]),
+    "mdtw_ed_fft": (C.c_int, [_P, _P, _I64, _I32, _P]),
     "mdtw_cdtw_backward": (C.c_int, [_P, _P, _P, _P, _P, _P, _I64, _I32, _I32, _I32, _P, _P,
                                      _P]),
     "mdtw_search": (C.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64,

@@ -22,12 +22,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import (ALGO_CDTW, ALGO_ED, ALGO_LDTW, METRIC_L1, METRIC_L2, STATS_FP64,
+from ._lib import (ALGO_CDTW, ALGO_ED, ALGO_ED_FFT, ALGO_LDTW, METRIC_L1, METRIC_L2, STATS_FP64,
                    STATS_NONE, STATS_REFCOMPAT, check)
 
 __all__ = ["Engine", "SearchResult", "Batch", "BatchResult", "znormalize", "envelope",
            "band_width",
-           "ALGO_CDTW", "ALGO_ED", "ALGO_LDTW", "METRIC_L1", "METRIC_L2",
+           "ALGO_CDTW", "ALGO_ED", "ALGO_ED_FFT", "ALGO_LDTW", "METRIC_L1", "METRIC_L2",
            "STATS_FP64", "STATS_NONE", "STATS_REFCOMPAT"]
 
 
@@ -237,6 +237,16 @@ class Engine:
         self._stream()
         check(self.lib.mdtw_ed(self.h, self._f32(series, "series"), self._f32(avg, "avg"),
                                self._f32(sd, "std"), self._f32(out, "out"), n), "mdtw_ed")
+        return out[:n]
+
+    def ed_fft(self, series, stats_mode: int = STATS_FP64):
+        """ed.cuh:71-153 (FFT cross-correlation, FP64): squared z-normalised
+        ED of every window, for long queries (M <= 8192)."""
+        n = series.numel() - self.m + 1
+        out = self.empty(max(n, 1))
+        self._stream()
+        check(self.lib.mdtw_ed_fft(self.h, self._f32(series, "series"), series.numel(),
+                                   int(stats_mode), self._f32(out, "out")), "mdtw_ed_fft")
         return out[:n]
 
     # -- gradient ----------------------------------------------------------

@@ -1,0 +1,495 @@
+// ed_fft.hip -- Euclidean distance of every window through FFT
+// cross-correlation (FP64), for long queries.
+//
+// The reference computes ED this way for every query (ed.cuh:71-153: cuFFT
+// of the stream and of the query in FP64, a product, an inverse, then
+// 2M - 2 corr / sigma with FP64 window statistics).  Direct ED costs 3M
+// VALU ops per window and wins for short queries; the FFT costs
+// O(log N) per window whatever M is, so it wins for long ones.
+//
+// Overlap-save in blocks of N real samples (N = the smallest power of two
+// >= 2M, at least 4096, at most 16384: M <= 8192): block b holds samples
+// [bV, bV + N) with V = N - M + 1 windows, minus the block's first sample c
+// (offset data loses no precision).  One workgroup of 1024 threads per
+// block, all in LDS (H = N/2 complex doubles, at most 128 KiB):
+//   - the N real samples are packed as H complex values z[n] = x[2n] +
+//     i x[2n+1], stored at bit-reversed positions,
+//   - forward radix-2 decimation-in-time FFT of size H, in place,
+//   - real-spectrum split X[k] = E[k] + W^k O[k], E/O from Z[k] and
+//     conj(Z[H-k]) (pairs k, H-k in one thread), product Y = X conj(Q)
+//     with the query's spectrum Q (same transform, computed once), and the
+//     inverse's packing A[k] + i B[k], A = Y[k] + conj(Y[H-k]),
+//     B = (Y[k] - conj(Y[H-k])) conj(W^k),
+//   - inverse radix-2 decimation-in-frequency FFT (natural order in,
+//     bit-reversed out), so y[2n] + i y[2n+1] = z'[n] / N is the
+//     cross-correlation dot_r = sum_i q_i (x_{bV+r+i} - c), r < V,
+//   - ED^2 = sum q^2 + sum z^2 - 2 (dot_r - (mu_r - c) sum q) / sigma_r,
+//     with the window's FP64 mean and variance (the statistics tile kernel
+//     in FP64): sum z^2 = M (0 for a constant window, sigma := 1).  Raw
+//     values are not offered here: sum q^2 + sum x^2 - 2 sum q x cancels
+//     about eleven digits on offset data.
+// Twiddles W^j = exp(-2 pi i j / N), j < N/2, come from sincospi (exact
+// argument 2j/N) into a table read through the cache.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace mdtw;
+
+namespace {
+
+constexpr int kFftThreads = 1024;
+constexpr int kFftMinLog = 12;  // N >= 4096 real samples per block
+constexpr int kFftMaxLog = 14;  // N <= 16384 (H = 8192 complex doubles in LDS)
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ double2 cconj(double2 a) { return {a.x, -a.y}; }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return {a.x * s, a.y * s}; }
+
+template <int LOGH>
+__device__ __forceinline__ int bitrev(int n) {
+    return static_cast<int>(__brev(static_cast<unsigned>(n)) >> (32 - LOGH));
+}
+
+__global__ __launch_bounds__(256) void twiddle_kernel(double2* __restrict__ tw, int logn) {
+    const int h = 1 << (logn - 1);
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= h) return;
+    double sn, cs;
+    sincospi(static_cast<double>(2 * j) / static_cast<double>(1 << logn), &sn, &cs);
+    tw[j] = {cs, -sn};
+}
+
+// In-place forward DIT FFT of a[0..H) (bit-reversed in, natural out).
+template <int LOGH>
+__device__ __forceinline__ void fft_dit(double2* a, const double2* __restrict__ tw) {
+    constexpr int H = 1 << LOGH;
+    for (int s = 1; s <= LOGH; ++s) {
+        const int half = 1 << (s - 1);
+        for (int b = threadIdx.x; b < H / 2; b += kFftThreads) {
+            const int pos = b & (half - 1);
+            const int i = ((b >> (s - 1)) << s) + pos;
+            const int j = i + half;
+            const double2 w = tw[pos << (LOGH - s + 1)];
+            const double2 u = a[i];
+            const double2 v = cmul(a[j], w);
+            a[i] = cadd(u, v);
+            a[j] = csub(u, v);
+        }
+        __syncthreads();
+    }
+}
+
+// In-place inverse DIF FFT of a[0..H) (natural in, bit-reversed out),
+// unnormalised: sum_k a[k] exp(+2 pi i n k / H).
+template <int LOGH>
+__device__ __forceinline__ void ifft_dif(double2* a, const double2* __restrict__ tw) {
+    constexpr int H = 1 << LOGH;
+    for (int s = LOGH; s >= 1; --s) {
+        const int half = 1 << (s - 1);
+        for (int b = threadIdx.x; b < H / 2; b += kFftThreads) {
+            const int pos = b & (half - 1);
+            const int i = ((b >> (s - 1)) << s) + pos;
+            const int j = i + half;
+            const double2 w = cconj(tw[pos << (LOGH - s + 1)]);
+            const double2 u = a[i];
+            const double2 v = a[j];
+            a[i] = cadd(u, v);
+            a[j] = cmul(csub(u, v), w);
+        }
+        __syncthreads();
+    }
+}
+
+// X[k] and X[H-k] of the real N-point spectrum from Z[k], Z[H-k] (1 <= k < H).
+__device__ __forceinline__ void real_split(double2 zk, double2 zm, double2 wk, double2 wm,
+                                           double2* xk, double2* xm) {
+    const double2 e = cscale(cadd(zk, cconj(zm)), 0.5);
+    const double2 d = csub(zk, cconj(zm));
+    const double2 o = {0.5 * d.y, -0.5 * d.x};  // d / (2i)
+    *xk = cadd(e, cmul(wk, o));
+    *xm = cadd(cconj(e), cmul(wm, cconj(o)));
+}
+
+// The query's real spectrum Q[0..H] (q zero-padded to N = 2H).
+template <int LOGH>
+__global__ __launch_bounds__(kFftThreads) void query_spectrum_kernel(
+    const float* __restrict__ q, int32_t m, const double2* __restrict__ tw,
+    double2* __restrict__ qs) {
+    constexpr int H = 1 << LOGH;
+    __shared__ double2 a[H];
+    for (int n = threadIdx.x; n < H; n += kFftThreads) {
+        const double v0 = 2 * n < m ? static_cast<double>(q[2 * n]) : 0.0;
+        const double v1 = 2 * n + 1 < m ? static_cast<double>(q[2 * n + 1]) : 0.0;
+        a[bitrev<LOGH>(n)] = {v0, v1};
+    }
+    __syncthreads();
+    fft_dit<LOGH>(a, tw);
+    for (int k = threadIdx.x; k <= H / 2; k += kFftThreads) {
+        if (k == 0) {
+            const double2 z0 = a[0];
+            qs[0] = {z0.x + z0.y, 0.0};
+            qs[H] = {z0.x - z0.y, 0.0};
+        } else {
+            double2 xk, xm;
+            real_split(a[k], a[H - k], tw[k], tw[H - k], &xk, &xm);
+            qs[k] = xk;
+            qs[H - k] = xm;  // k = H/2: the same value twice
+        }
+    }
+}
+
+struct FftScalars {
+    int64_t L;        // windows of the stream
+    int64_t n;        // stream samples
+    int64_t V;        // windows per block
+    int32_t m;
+    int32_t pad;
+    double q2;        // sum of q^2
+    double qsum;      // sum of q
+};
+
+template <int LOGH>
+__global__ __launch_bounds__(kFftThreads) void ed_fft_kernel(
+    const float* __restrict__ s, const double* __restrict__ avg, const double* __restrict__ var,
+    const double2* __restrict__ tw, const double2* __restrict__ qs, float* __restrict__ out,
+    FftScalars a) {
+    constexpr int H = 1 << LOGH;
+    constexpr int N = 2 * H;
+    __shared__ double2 z[H];
+    const int64_t base = static_cast<int64_t>(blockIdx.x) * a.V;
+    const double c = static_cast<double>(s[base]);
+    for (int n = threadIdx.x; n < H; n += kFftThreads) {
+        const int64_t i0 = base + 2 * n;
+        const double v0 = i0 < a.n ? static_cast<double>(s[i0]) - c : 0.0;
+        const double v1 = i0 + 1 < a.n ? static_cast<double>(s[i0 + 1]) - c : 0.0;
+        z[bitrev<LOGH>(n)] = {v0, v1};
+    }
+    __syncthreads();
+    fft_dit<LOGH>(z, tw);
+    for (int k = threadIdx.x; k <= H / 2; k += kFftThreads) {
+        if (k == 0) {
+            const double2 z0 = z[0];
+            const double2 y0 = cmul({z0.x + z0.y, 0.0}, cconj(qs[0]));
+            const double2 yh = cmul({z0.x - z0.y, 0.0}, cconj(qs[H]));
+            const double2 A = cadd(y0, cconj(yh));
+            const double2 B = csub(y0, cconj(yh));
+            z[0] = {A.x - B.y, A.y + B.x};
+        } else {
+            const int km = H - k;
+            const double2 wk = tw[k], wm = tw[km];
+            double2 xk, xm;
+            real_split(z[k], z[km], wk, wm, &xk, &xm);
+            const double2 yk = cmul(xk, cconj(qs[k]));
+            const double2 ym = cmul(xm, cconj(qs[km]));
+            const double2 Ak = cadd(yk, cconj(ym));
+            const double2 Bk = cmul(csub(yk, cconj(ym)), cconj(wk));
+            z[k] = {Ak.x - Bk.y, Ak.y + Bk.x};
+            if (km != k) {
+                const double2 Am = cadd(ym, cconj(yk));
+                const double2 Bm = cmul(csub(ym, cconj(yk)), cconj(wm));
+                z[km] = {Am.x - Bm.y, Am.y + Bm.x};
+            }
+        }
+    }
+    __syncthreads();
+    ifft_dif<LOGH>(z, tw);
+    const double inv_n = 1.0 / static_cast<double>(N);
+    const double m = static_cast<double>(a.m);
+    for (int r = threadIdx.x; r < a.V; r += kFftThreads) {
+        const int64_t p = base + r;
+        if (p >= a.L) break;
+        const double2 v = z[bitrev<LOGH>(r >> 1)];
+        const double dot = ((r & 1) ? v.y : v.x) * inv_n;  // sum q (x - c)
+        const double mu = avg[p], va = var[p];
+        const double sig = va > 0.0 ? sqrt(va) : 1.0;
+        const double sz2 = va > 0.0 ? m : 0.0;
+        const double ed2 = a.q2 + sz2 - 2.0 * (dot - (mu - c) * a.qsum) / sig;
+        out[p] = static_cast<float>(ed2 > 0.0 ? ed2 : 0.0);
+    }
+}
+
+// ---- selection over a distance profile --------------------------------------
+__global__ __launch_bounds__(256) void profile_best_kernel(const float* __restrict__ v, int64_t n,
+                                                           Ctrl* ctrl) {
+    unsigned long long best = ~0ull;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned long long key = pack_key(v[i], static_cast<uint32_t>(i));
+        best = key < best ? key : best;
+    }
+    best = wave_min_u64(best);
+    if ((threadIdx.x & (kWave - 1)) == 0 && best != ~0ull) atomicMin(&ctrl->best, best);
+}
+
+__global__ __launch_bounds__(256) void profile_gather_kernel(const float* __restrict__ v,
+                                                             int64_t count, int64_t stride,
+                                                             float* __restrict__ seeds) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < count) seeds[j] = v[j * stride];
+}
+
+// Keys (distance, index) of every entry at or below ctrl->thr.
+__global__ __launch_bounds__(256) void profile_collect_kernel(const float* __restrict__ v,
+                                                              int64_t n, Ctrl* ctrl,
+                                                              unsigned long long* __restrict__ res) {
+    const float thr = load_thr(ctrl);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool keep = i < n && v[i] <= thr;
+        const unsigned long long km = __ballot(keep);
+        if (!km) continue;
+        unsigned long long rb = 0;
+        if (lane == 0) rb = atomicAdd(&ctrl->nres, static_cast<unsigned long long>(__popcll(km)));
+        rb = __shfl(rb, 0, kWave);
+        if (keep) {
+            const uint32_t rank =
+                __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(km >> 32),
+                                          __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(km), 0u));
+            res[rb + rank] = pack_key(v[i], static_cast<uint32_t>(i));
+        }
+    }
+}
+
+int hip_ok(hipError_t err) {
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+int fft_log(int32_t m) {
+    int lg = kFftMinLog;
+    while ((int64_t(1) << lg) < 2 * static_cast<int64_t>(m)) ++lg;
+    return lg;
+}
+
+template <int LOGH>
+int launch_fft_pair(hipStream_t st, const float* s, const double* avg, const double* var,
+                    const double2* tw, const float* q, double2* qs, float* out,
+                    const FftScalars& a) {
+    hipLaunchKernelGGL(query_spectrum_kernel<LOGH>, dim3(1), dim3(kFftThreads), 0, st, q, a.m, tw,
+                       qs);
+    int rc = hip_ok(hipGetLastError());
+    if (rc != MDTW_OK) return rc;
+    const int64_t nb = (a.L + a.V - 1) / a.V;
+    hipLaunchKernelGGL(ed_fft_kernel<LOGH>, dim3(static_cast<unsigned>(nb)), dim3(kFftThreads), 0,
+                       st, s, avg, var, tw, qs, out, a);
+    return hip_ok(hipGetLastError());
+}
+
+constexpr int64_t kFftSeedTarget = 8192;
+
+// Sum of q^2 and of q over the engine's query (its first m host entries).
+void query_sums(const mdtw_engine* e, double* q2, double* qsum) {
+    double a = 0.0, b = 0.0;
+    for (int32_t i = 0; i < e->m; ++i) {
+        const double v = static_cast<double>(e->h_q[static_cast<size_t>(i)]);
+        a += v * v;
+        b += v;
+    }
+    *q2 = a;
+    *qsum = b;
+}
+
+}  // namespace
+
+namespace mdtw {
+
+size_t ed_fft_ws_bytes(int64_t L, int32_t m, int32_t k) {
+    const int lg = fft_log(m);
+    const int64_t h = int64_t(1) << (lg - 1);
+    return carve_size({sizeof(double) * L, sizeof(double) * L, sizeof(float) * L,
+                       sizeof(double2) * h, sizeof(double2) * (h + 1),
+                       sizeof(float) * kSeedMaxCount,
+                       k > 1 ? sizeof(unsigned long long) * L : 0});
+}
+
+// ED^2 profile of all L windows into out (ws: ed_fft_ws_bytes(L, m, 1)).
+int enqueue_ed_fft(hipStream_t st, const float* s, int64_t n, const float* q, int32_t m,
+                   double q2, double qsum, int32_t stats_mode, char* ws, float* out,
+                   float** prof_out) {
+    if (m < 1 || m > MDTW_MAX_QUERY) return MDTW_ERR_QUERY_TOO_LONG;
+    if (stats_mode != MDTW_STATS_FP64) return MDTW_ERR_UNSUPPORTED;
+    const int64_t L = n - m + 1;
+    if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
+    const int lg = fft_log(m);
+    if (lg > kFftMaxLog) return MDTW_ERR_UNSUPPORTED;
+    const int64_t h = int64_t(1) << (lg - 1);
+    Carve cv{ws};
+    double* avg = cv.take<double>(L);
+    double* var = cv.take<double>(L);
+    float* prof = cv.take<float>(L);
+    double2* tw = cv.take<double2>(h);
+    double2* qs = cv.take<double2>(h + 1);
+    if (!out) out = prof;
+    if (prof_out) *prof_out = out;
+    int rc = launch_stats_f64out(st, s, n, m, avg, var);
+    if (rc != MDTW_OK) return rc;
+    hipLaunchKernelGGL(twiddle_kernel, dim3(static_cast<unsigned>((h + 255) / 256)), dim3(256), 0,
+                       st, tw, lg);
+    rc = hip_ok(hipGetLastError());
+    if (rc != MDTW_OK) return rc;
+    FftScalars a;
+    a.L = L;
+    a.n = n;
+    a.V = (int64_t(1) << lg) - m + 1;
+    a.m = m;
+    a.pad = 0;
+    a.q2 = q2;
+    a.qsum = qsum;
+    switch (lg) {
+        case 12: return launch_fft_pair<11>(st, s, avg, var, tw, q, qs, out, a);
+        case 13: return launch_fft_pair<12>(st, s, avg, var, tw, q, qs, out, a);
+        default: return launch_fft_pair<13>(st, s, avg, var, tw, q, qs, out, a);
+    }
+}
+
+}  // namespace mdtw
+
+extern "C" {
+
+int mdtw_ed_fft(mdtw_engine* e, const float* d_series, int64_t n, int32_t stats_mode,
+                float* d_out) {
+    if (!e || !d_series || !d_out) return MDTW_ERR_INVALID_ARGUMENT;
+    if (!e->has_query) return MDTW_ERR_NO_QUERY;
+    const int32_t m = e->m;
+    if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
+    if (fft_log(m) > kFftMaxLog) return MDTW_ERR_UNSUPPORTED;
+    DeviceGuard guard(e->device);
+    const int64_t L = n - m + 1;
+    void* ws = nullptr;
+    int rc = ws_reserve(e, ed_fft_ws_bytes(L, m, 1), &ws);
+    if (rc != MDTW_OK) return rc;
+    double q2 = 0.0, qsum = 0.0;
+    query_sums(e, &q2, &qsum);
+    return enqueue_ed_fft(e->stream, d_series, n, e->d_q, m, q2, qsum, stats_mode,
+                          static_cast<char*>(ws), d_out, nullptr);
+}
+
+}  // extern "C"
+
+namespace mdtw {
+
+// Best / top-k over the FFT ED profile (mdtw_search with MDTW_ALGO_ED_FFT).
+int search_ed_fft(mdtw_engine* e, const float* d_series, int64_t n, int32_t stats_mode, int32_t k,
+                  int64_t excl, int64_t* h_pos, float* h_dist, mdtw_search_info* info) {
+    const int32_t m = e->m;
+    const int64_t L = n - m + 1;
+    void* wsv = nullptr;
+    int rc = ws_reserve(e, ed_fft_ws_bytes(L, m, k), &wsv);
+    if (rc != MDTW_OK) return rc;
+    char* ws = static_cast<char*>(wsv);
+    double q2 = 0.0, qsum = 0.0;
+    query_sums(e, &q2, &qsum);
+    hipStream_t st = e->stream;
+    rc = hip_ok(hipEventRecord(e->ev0, st));
+    if (rc == MDTW_OK) rc = launch_ctrl_reset(st, e->d_ctrl, kBestNone);
+    float* prof = nullptr;
+    if (rc == MDTW_OK)
+        rc = enqueue_ed_fft(st, d_series, n, e->d_q, m, q2, qsum, stats_mode, ws, nullptr, &prof);
+    if (rc != MDTW_OK) return rc;
+    // The profile's scratch layout, as enqueue_ed_fft carved it.
+    const int lg = fft_log(m);
+    const int64_t h = int64_t(1) << (lg - 1);
+    Carve cv{ws};
+    (void)cv.take<double>(L);
+    (void)cv.take<double>(L);
+    (void)cv.take<float>(L);
+    (void)cv.take<double2>(h);
+    (void)cv.take<double2>(h + 1);
+    float* seeds = cv.take<float>(kSeedMaxCount);
+    unsigned long long* res = k > 1 ? cv.take<unsigned long long>(L) : nullptr;
+    const unsigned nbl = static_cast<unsigned>(
+        std::min<int64_t>((L + 255) / 256, static_cast<int64_t>(e->cus) * 16));
+    int64_t seeded = 0;
+    if (k == 1) {
+        hipLaunchKernelGGL(profile_best_kernel, dim3(nbl), dim3(256), 0, st, prof, L, e->d_ctrl);
+        rc = hip_ok(hipGetLastError());
+    } else {
+        // k-th smallest of entries >= 2*excl apart bounds the k-th greedy
+        // pick; every entry at or below it is kept and picked on the host.
+        int64_t stride = std::max<int64_t>(std::max<int64_t>(1, L / kFftSeedTarget),
+                                           2 * std::max<int64_t>(excl, 0));
+        int64_t count = (L - 1) / stride + 1;
+        if (count > kSeedMaxCount) {
+            stride = std::max<int64_t>((L + kSeedMaxCount - 1) / kSeedMaxCount,
+                                       2 * std::max<int64_t>(excl, 0));
+            count = (L - 1) / stride + 1;
+        }
+        seeded = count;
+        hipLaunchKernelGGL(profile_gather_kernel, dim3(static_cast<unsigned>((count + 255) / 256)),
+                           dim3(256), 0, st, prof, count, stride, seeds);
+        rc = hip_ok(hipGetLastError());
+        if (rc == MDTW_OK) rc = launch_seed_threshold(st, seeds, count, k, e->d_ctrl, INFINITY);
+        if (rc == MDTW_OK) {
+            hipLaunchKernelGGL(profile_collect_kernel, dim3(nbl), dim3(256), 0, st, prof, L,
+                               e->d_ctrl, res);
+            rc = hip_ok(hipGetLastError());
+        }
+    }
+    if (rc == MDTW_OK) rc = hip_ok(hipEventRecord(e->ev1, st));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(e->h_ctrl, e->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+    if (rc == MDTW_OK) rc = hip_ok(hipStreamSynchronize(st));
+    if (rc != MDTW_OK) return rc;
+    const Ctrl c = *e->h_ctrl;
+    int32_t found = 0;
+    float thr_out = INFINITY;
+    if (k == 1) {
+        if ((c.best >> 32) < 0x7f800000ull) {
+            const uint32_t bits = static_cast<uint32_t>(c.best >> 32);
+            float d;
+            std::memcpy(&d, &bits, sizeof(d));
+            h_pos[0] = static_cast<int64_t>(c.best & 0xffffffffull);
+            h_dist[0] = std::sqrt(d);
+            found = 1;
+            thr_out = d;
+        }
+    } else {
+        thr_out = c.thr;
+        std::vector<unsigned long long> keys(static_cast<size_t>(c.nres));
+        if (c.nres) {
+            rc = hip_ok(hipMemcpyAsync(keys.data(), res, sizeof(unsigned long long) * c.nres,
+                                       hipMemcpyDeviceToHost, st));
+            if (rc == MDTW_OK) rc = hip_ok(hipStreamSynchronize(st));
+            if (rc != MDTW_OK) return rc;
+        }
+        std::sort(keys.begin(), keys.end());
+        std::vector<std::pair<float, int64_t>> v;
+        v.reserve(keys.size());
+        for (unsigned long long key : keys) {
+            const uint32_t bits = static_cast<uint32_t>(key >> 32);
+            float d;
+            std::memcpy(&d, &bits, sizeof(d));
+            v.emplace_back(d, static_cast<int64_t>(key & 0xffffffffull));
+        }
+        found = greedy_pick(v, k, excl, MDTW_METRIC_L2, h_pos, h_dist);
+    }
+    if (info) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
+        info->windows = L;
+        info->seeded = seeded;
+        info->lb_kim_pruned = 0;
+        info->lb_keogh_pruned = 0;
+        info->dp_evaluated = L;
+        info->dp_abandoned = 0;
+        info->found = found;
+        info->threshold = thr_out;
+        info->elapsed_ms = ms;
+    }
+    return MDTW_OK;
+}
+
+}  // namespace mdtw

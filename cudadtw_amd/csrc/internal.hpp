@@ -225,6 +225,11 @@ int64_t search_seed_count(int64_t L, int32_t k, int64_t excl);
 // Enqueues the whole search on p.st (no host synchronisation, no allocation;
 // safe to capture in a graph when stats_mode is FP64).
 int enqueue_search(const SearchPlan& p, SearchBufs* bufs);
+// FFT ED (ed_fft.hip): scratch of a profile (k = 1) or a top-k search, and
+// the search itself (statistics MDTW_STATS_FP64 only).
+size_t ed_fft_ws_bytes(int64_t L, int32_t m, int32_t k);
+int search_ed_fft(mdtw_engine* e, const float* d_series, int64_t n, int32_t stats_mode, int32_t k,
+                  int64_t excl, int64_t* h_pos, float* h_dist, mdtw_search_info* info);
 // Greedy top-k with exclusion over the kept keys on the device (one
 // workgroup): out_pos/out_dist get k entries (position -1 / +inf past the
 // matches found); distances are sqrt'ed for L2.  k = 1 decodes ctrl->best.
@@ -276,6 +281,10 @@ int launch_bwd_reduce_batch(hipStream_t st, const float* gpair, const BwdQuery* 
 // stats.hip
 int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,
                       float* avg, float* sd);
+// FP64 mean and variance (0 for a constant window) of every window: the same
+// tile sums as launch_stats_fp64, kept in FP64.
+int launch_stats_f64out(hipStream_t st, const float* s, int64_t n, int32_t m, double* avg,
+                        double* var);
 // MDTW_STATS_NONE: avg[0..count) = 0, sd[0..count) = 1.
 int launch_stats_none(hipStream_t st, int64_t count, float* avg, float* sd);
 int launch_stats_refcompat(mdtw_engine* e, const float* s, int64_t n, int32_t m,

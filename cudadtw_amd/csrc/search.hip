@@ -304,11 +304,16 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
                                                                     : MDTW_ERR_INVALID_ARGUMENT;
     if (!e->has_query) return MDTW_ERR_NO_QUERY;
     if ((d_avg == nullptr) != (d_std == nullptr)) return MDTW_ERR_INVALID_ARGUMENT;
-    if (algo != MDTW_ALGO_CDTW && algo != MDTW_ALGO_ED && algo != MDTW_ALGO_LDTW)
+    if (algo != MDTW_ALGO_CDTW && algo != MDTW_ALGO_ED && algo != MDTW_ALGO_LDTW &&
+        algo != MDTW_ALGO_ED_FFT)
         return MDTW_ERR_INVALID_ARGUMENT;
     if (k < 1 || k > MDTW_MAX_K) return MDTW_ERR_K;
     if (metric != MDTW_METRIC_L1 && metric != MDTW_METRIC_L2) return MDTW_ERR_INVALID_ARGUMENT;
-    if (algo == MDTW_ALGO_ED && metric != MDTW_METRIC_L2) return MDTW_ERR_UNSUPPORTED;
+    if ((algo == MDTW_ALGO_ED || algo == MDTW_ALGO_ED_FFT) && metric != MDTW_METRIC_L2)
+        return MDTW_ERR_UNSUPPORTED;
+    // The FFT form needs FP64 window statistics, which it computes itself.
+    if (algo == MDTW_ALGO_ED_FFT && (d_avg != nullptr || stats_mode != MDTW_STATS_FP64))
+        return MDTW_ERR_UNSUPPORTED;
     if (stats_mode != MDTW_STATS_FP64 && stats_mode != MDTW_STATS_REFCOMPAT &&
         stats_mode != MDTW_STATS_NONE)
         return MDTW_ERR_INVALID_ARGUMENT;
@@ -323,6 +328,8 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
         h_dist[i] = INFINITY;
     }
     DeviceGuard guard(e->device);
+    if (algo == MDTW_ALGO_ED_FFT)
+        return search_ed_fft(e, d_series, n, stats_mode, k, excl, h_pos, h_dist, info);
 
     // Query-side state first (host work and small uploads, outside timing).
     int rc = MDTW_OK;

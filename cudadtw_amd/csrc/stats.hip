@@ -140,8 +140,11 @@ __device__ __forceinline__ void stats_wave_fence() {
 // leaves mean and sigma of window w0 + i in xa / xb at stats_skew(i).  The
 // whole-stream kernel and the window-list kernel below both run exactly this,
 // so a window's statistics do not depend on which of them computed it.
+// T = double (the FFT ED's statistics): the same sums, leaving the FP64 mean
+// and the FP64 variance (0 for a constant window) instead.
+template <class T = float>
 __device__ __forceinline__ int stats_tile(const float* __restrict__ s, int64_t L, int32_t m,
-                                          int64_t w0, float* xa, float* xb) {
+                                          int64_t w0, T* xa, T* xb) {
     const int lane = threadIdx.x & (kWave - 1);
     const int nw = L - w0 < kStatsTW ? static_cast<int>(L - w0) : kStatsTW;
     const int64_t last = L - 1 + m - 1;  // last stream sample
@@ -218,10 +221,15 @@ __device__ __forceinline__ int stats_tile(const float* __restrict__ s, int64_t L
         const int k = stats_skew(r);
         const double xo = static_cast<double>(xa[k]) - c;
         const double xn = static_cast<double>(xb[k]) - c;
-        xa[k] = static_cast<float>(c + mean);
-        // FP32 square root of the FP64 variance rounded to FP32: within one
-        // ulp of the FP64 root's rounding, at a fraction of its cost.
-        xb[k] = var > 0.0 ? sqrtf(static_cast<float>(var)) : 1.0f;
+        if constexpr (sizeof(T) == sizeof(double)) {
+            xa[k] = c + mean;
+            xb[k] = var;
+        } else {
+            xa[k] = static_cast<float>(c + mean);
+            // FP32 square root of the FP64 variance rounded to FP32: within
+            // one ulp of the FP64 root's rounding, at a fraction of its cost.
+            xb[k] = var > 0.0 ? sqrtf(static_cast<float>(var)) : 1.0f;
+        }
         p1 += xn - xo;
         p2 += xn * xn - xo * xo;
         q1 += fabs(xn) + fabs(xo);
@@ -249,6 +257,29 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
         if (i < nw) {  // streamed out: not re-read by this kernel
             __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
             __builtin_nontemporal_store(xb[stats_skew(i)], sd + w0 + i);
+        }
+    }
+}
+
+// FP64 mean and variance of every window (the FFT ED's statistics).
+__global__ __launch_bounds__(kStatsWaves * kWave) void stats_f64out_kernel(
+    const float* __restrict__ s, int64_t L, int32_t m, double* __restrict__ avg,
+    double* __restrict__ var) {
+    __shared__ double xa_all[kStatsWaves][kStatsLds];
+    __shared__ double xb_all[kStatsWaves][kStatsLds];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    double* xa = xa_all[wid];
+    double* xb = xb_all[wid];
+    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
+    if (w0 >= L) return;
+    const int nw = stats_tile<double>(s, L, m, w0, xa, xb);
+#pragma unroll 4
+    for (int v = 0; v < kStatsR; ++v) {
+        const int i = v * kWave + lane;
+        if (i < nw) {
+            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
+            __builtin_nontemporal_store(xb[stats_skew(i)], var + w0 + i);
         }
     }
 }
@@ -399,6 +430,22 @@ int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m, floa
     const int64_t nb = (L + per_block - 1) / per_block;
     hipLaunchKernelGGL(stats_fp64_kernel, dim3(static_cast<unsigned>(nb)),
                        dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+int launch_stats_f64out(hipStream_t st, const float* s, int64_t n, int32_t m, double* avg,
+                        double* var) {
+    const int64_t L = n - m + 1;
+    if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
+    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
+    const int64_t nb = (L + per_block - 1) / per_block;
+    hipLaunchKernelGGL(stats_f64out_kernel, dim3(static_cast<unsigned>(nb)),
+                       dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, var);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

@@ -136,7 +136,8 @@ struct mdtw_stream {
 namespace {
 
 uint64_t query_hash_of(const mdtw_engine* e) {
-    return fnv1a(e->h_q.data(), sizeof(float) * e->h_q.size(),
+    // the first m entries: the host copy is sized for the longest query
+    return fnv1a(e->h_q.data(), sizeof(float) * static_cast<size_t>(e->m),
                  fnv1a(&e->m, sizeof(e->m)));
 }
 

@@ -68,6 +68,9 @@ extern "C" {
 #define MDTW_ALGO_CDTW 0  /* cdtw.cuh:510-592 prune_cdtw            */
 #define MDTW_ALGO_ED   1  /* ed.cuh:71-153 calculate_ed            */
 #define MDTW_ALGO_LDTW 2  /* ldtw.cuh:294-391 ztsh2_ldtw + argmin  */
+#define MDTW_ALGO_ED_FFT 3 /* ed.cuh:71-153 calculate_ed's FFT form: FP64
+                              cross-correlation, for long queries (M <= 8192;
+                              statistics MDTW_STATS_FP64, metric L2) */
 
 typedef struct mdtw_engine mdtw_engine;
 
@@ -149,6 +152,13 @@ int mdtw_ldtw(mdtw_engine* e, const float* d_series, const float* d_avg,
  * (the reference stores 2M-shifted values, ed.cuh:68; this stores ED^2). */
 int mdtw_ed(mdtw_engine* e, const float* d_series, const float* d_avg,
             const float* d_std, float* d_out, int64_t count);
+
+/* ed.cuh:71-153 (calculate_ed's FFT cross-correlation, FP64): squared
+ * z-normalised ED of every window 0..n-M, into d_out[n-M+1], with FP64 window
+ * statistics computed here (stats_mode MDTW_STATS_FP64 only).  For M up to
+ * 8192; faster than mdtw_ed for long queries. */
+int mdtw_ed_fft(mdtw_engine* e, const float* d_series, int64_t n,
+                int32_t stats_mode, float* d_out);
 
 /* No reference counterpart: the same profile as mdtw_cdtw, forced through the
  * anti-diagonal kernel that serves bands wider than the register-band kernel

@@ -169,3 +169,24 @@ def test_stream_argument_errors(engine):
     st.finish()
     same(st.result(), want)
     st.close()
+
+
+def test_restore_after_longer_queries_in_between(engine):
+    # The state checks the query by hash over its own samples only: setting
+    # longer queries before and after must not change it.
+    q = O.znormalize_f32(synth.random_walk(64, 11))
+    engine.set_query(O.znormalize_f32(synth.random_walk(900, 12)), 30)
+    engine.set_query(q, 6)
+    s = synth.ecg_like(120_000, 13)
+    want = one_shot(engine, s, 6, k=3, excl=32)
+    st = engine.stream(6, k=3, excl=32, epoch_windows=EPOCH)
+    st.push(s[:70_000])
+    state = st.save()
+    st.close()
+    engine.set_query(O.znormalize_f32(synth.random_walk(2000, 14)), 100)
+    engine.set_query(q, 6)
+    st = engine.restore_stream(state)
+    st.push(s[70_000:])
+    st.finish()
+    same(st.result(), want)
+    st.close()
