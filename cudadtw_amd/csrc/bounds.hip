@@ -192,11 +192,13 @@ constexpr int kRing = 256;
 // a time: one atomicAdd on the shared candidate counter per flush instead of
 // one per 64 windows (same-address atomics serialise in L2, ~15 ns each).
 // The last flush and the prune counters are summed per block first: three
-// atomics per block, not per wave, and every wave gets at least
-// kCascadeMinChunks chunks of 64 windows, so a short range (a streaming
-// epoch) does not pay the atomics of a grid sized for a long one.
+// atomics per block, not per wave.  A wave takes kCascadeChunks chunks of 64
+// windows where the range fills the chip twice over with that many, so a
+// medium range (a streaming epoch) does not pay the atomics of a grid sized
+// for a long one; a shorter range gets fewer chunks per wave (down to one),
+// because there the waves' serial latency, not the atomics, sets the time.
 constexpr int kCandBuf = 512;
-constexpr int64_t kCascadeMinChunks = 16;
+constexpr int64_t kCascadeChunks = 16;
 
 __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -384,12 +386,16 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg, const floa
 int launch_cascade(hipStream_t st, const float* s, const float* avg, const float* sd,
                    const float* q, const float* lo, const float* up, int32_t m, int64_t base,
                    int64_t L, int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
-                   float* cand_lb, int64_t max_blocks) {
+                   float* cand_lb, int64_t max_blocks, int cus) {
     if (L <= 0) return MDTW_OK;
-    // Enough waves to cover the chip several times over, each with at least
-    // kCascadeMinChunks chunks of 64 windows; each wave strides.
+    // Chunks of 64 windows per wave: kCascadeChunks once the range gives two
+    // waves per SIMD with that many, fewer on shorter ranges; the grid is
+    // capped at max_blocks and each wave strides.
     const int64_t chunks = (L + kWave - 1) / kWave;
-    const int64_t waves_needed = (chunks + kCascadeMinChunks - 1) / kCascadeMinChunks;
+    const int64_t target_waves = 2 * kCascadeWaves * static_cast<int64_t>(cus > 0 ? cus : 1);
+    const int64_t cpw = std::min<int64_t>(
+        kCascadeChunks, std::max<int64_t>(1, (chunks + target_waves - 1) / target_waves));
+    const int64_t waves_needed = (chunks + cpw - 1) / cpw;
     const int64_t nb64 = std::max<int64_t>(
         1, std::min<int64_t>((waves_needed + kCascadeWaves - 1) / kCascadeWaves, max_blocks));
     const unsigned nb = static_cast<unsigned>(nb64);

@@ -180,7 +180,10 @@ __global__ __launch_bounds__(64) void wide_kernel(
         if (lane == 0) b = atomicAdd(&ctrl->work, 1ull);
         return static_cast<int64_t>(__shfl(b, 0, kWave));
     };
-    int64_t g = a.persistent ? next() : static_cast<int64_t>(blockIdx.x);
+    // First window by grid position; persistent waves then take windows
+    // beyond the grid's first pass from the cursor (none when it covers all).
+    const int64_t pass = static_cast<int64_t>(gridDim.x);
+    int64_t g = static_cast<int64_t>(blockIdx.x);
     while (g < count) {
         const int64_t w = idx ? static_cast<int64_t>(idx[g]) : a.base + g * a.stride;
         bool alive = true;
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(64) void wide_kernel(
         n_dp += 1;
         n_ab += alive ? 0 : 1;
         if (!a.persistent) break;
-        g = next();
+        g = pass >= count ? count : pass + next();
     }
     if (lane == 0 && a.mode != kModeProfile) {
         if (n_dp) atomicAdd(&ctrl->dp, n_dp);

@@ -283,11 +283,16 @@ __global__ __launch_bounds__(256) void band_kernel(
         if (threadIdx.x == 0) drained = 0;
         __syncthreads();
     }
-    // persistent: waves take 64 windows at a time from an atomic cursor;
-    // otherwise a static grid stride (every wave the same amount of work).
+    // Every wave's first 64 windows are fixed by its position in the grid;
+    // after that, persistent waves take 64 windows at a time from an atomic
+    // cursor over the windows beyond the grid's first pass, and the others
+    // stride by the grid (every wave the same amount of work).  A persistent
+    // wave whose first chunk is already past the end, or a grid that covers
+    // every window in one pass, touches the cursor not at all: same-address
+    // atomics serialise (~15 ns each), which for a short candidate list and a
+    // grid sized for the chip would otherwise cost more than the DP.
     const int64_t step = static_cast<int64_t>(gridDim.x) * kBandWaves * kWave;
-    int64_t base = a.persistent ? band_next(ctrl, lane, count, &drained)
-                                : (static_cast<int64_t>(blockIdx.x) * kBandWaves + wid) * kWave;
+    int64_t base = (static_cast<int64_t>(blockIdx.x) * kBandWaves + wid) * kWave;
     while (base < count) {
         const int64_t g = base + lane;
         const bool valid = g < count;
@@ -319,7 +324,10 @@ __global__ __launch_bounds__(256) void band_kernel(
         }
         n_dp += __popcll(__ballot(valid));
         n_ab += __popcll(__ballot(valid && !alive));
-        base = a.persistent ? band_next(ctrl, lane, count, &drained) : base + step;
+        if (!a.persistent)
+            base += step;
+        else
+            base = step >= count ? count : step + band_next(ctrl, lane, count - step, &drained);
     }
     // Counters summed per block: two atomics per block, not per wave.
     if (a.mode != kModeProfile) {

@@ -308,7 +308,8 @@ int launch_cascade(hipStream_t st, const float* s, const float* avg,
                    const float* sd, const float* q, const float* lo,
                    const float* up, int32_t m, int64_t base, int64_t count,
                    int32_t metric, bool use_best, Ctrl* ctrl, int32_t* cand,
-                   float* cand_lb = nullptr, int64_t max_blocks = 256 * 64);
+                   float* cand_lb = nullptr, int64_t max_blocks = 256 * 64,
+                   int cus = 256);
 // cdtw.hip: band kernel (qx = padded query) or, for wide bands, the
 // anti-diagonal kernel (q_plain).
 int launch_dp(hipStream_t st, int cus, const BandArgs& a, const float* q_plain,

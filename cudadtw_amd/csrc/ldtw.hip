@@ -104,8 +104,10 @@ __global__ __launch_bounds__(64) void ldtw_kernel(
         count = static_cast<int64_t>(
             __hip_atomic_load(&ctrl->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     unsigned long long n_dp = 0, n_ab = 0;
-    int64_t base = a.persistent ? band_next(ctrl, lane)
-                                : static_cast<int64_t>(blockIdx.x) * kWave;
+    // First chunk by grid position; persistent waves then take chunks beyond
+    // the grid's first pass from the cursor (none when the grid covers all).
+    const int64_t pass = static_cast<int64_t>(gridDim.x) * kWave;
+    int64_t base = static_cast<int64_t>(blockIdx.x) * kWave;
     while (base < count) {
         const int64_t g = base + lane;
         const bool valid = g < count;
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(64) void ldtw_kernel(
         n_dp += __popcll(__ballot(valid));
         n_ab += __popcll(__ballot(valid && !alive));
         if (!a.persistent) break;
-        base = band_next(ctrl, lane);
+        base = pass >= count ? count : pass + band_next(ctrl, lane);
     }
     if (lane == 0 && a.mode != kModeProfile) {
         if (n_dp) atomicAdd(&ctrl->dp, n_dp);

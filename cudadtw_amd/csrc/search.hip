@@ -267,7 +267,8 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
             rc = launch_round_reset(st, p.ctrl);
             if (rc == MDTW_OK)
                 rc = launch_cascade(st, p.s, avg, sd, p.q, p.lo, p.up, m, r.base, r.count,
-                                    p.metric, !topk, p.ctrl, cand, cand_lb, cascade_cap);
+                                    p.metric, !topk, p.ctrl, cand, cand_lb, cascade_cap,
+                                    p.cus);
             if (rc == MDTW_OK) {
                 BandArgs da = a;
                 da.idx = cand;

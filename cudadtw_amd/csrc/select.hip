@@ -117,33 +117,103 @@ __global__ __launch_bounds__(256) void thr_scatter_kernel(const float* __restric
     }
 }
 
-// k-th smallest of n (<= kSeedMax) values: bitonic sort in LDS.
-constexpr int kSeedMax = 16384;
+// Order-preserving map of float bits to unsigned keys (negative values
+// reversed, the sign bit set on the rest), and its inverse.
+__device__ __forceinline__ uint32_t ord_key(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float unord_key(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
 
-__global__ __launch_bounds__(1024) void seed_thr_kernel(const float* __restrict__ v, int n, int k,
-                                                        float cap, Ctrl* ctrl) {
-    __shared__ float a[kSeedMax];
-    int size = 1;
-    while (size < n) size <<= 1;
-    for (int i = threadIdx.x; i < size; i += blockDim.x) a[i] = i < n ? v[i] : kInf;
-    __syncthreads();
-    for (int len = 2; len <= size; len <<= 1) {
-        for (int st = len >> 1; st > 0; st >>= 1) {
-            for (int i = threadIdx.x; i < size; i += blockDim.x) {
-                const int j = i ^ st;
-                if (j > i) {
-                    const bool up = (i & len) == 0;
-                    const float x = a[i], y = a[j];
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[j] = x;
-                    }
-                }
+// One wave finds the bin of rank krem (1-based) in hist[0..64*PER): lane l
+// sums bins [PER*l, PER*l + PER), a wave prefix sum picks the lane, the lane
+// walks its bins.  Returns (bin, rank inside the bin) to every lane, or bin
+// 0xffffffff when the bins hold fewer than krem values.
+template <int PER>
+__device__ __forceinline__ void wave_find_rank(const uint32_t* hist, uint32_t krem,
+                                               uint32_t& bin, uint32_t& rank_in) {
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) s += hist[lane * PER + i];
+    uint32_t incl = s;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += o;
+    }
+    const unsigned long long hit = __ballot(incl >= krem);
+    bin = 0xffffffffu;
+    rank_in = 0;
+    if (hit == 0ull) return;
+    const int L = __ffsll(static_cast<long long>(hit)) - 1;
+    uint32_t b = 0xffffffffu, r = 0;
+    if (lane == L) {
+        uint32_t run = incl - s;
+        for (int i = 0; i < PER; ++i) {
+            const uint32_t c = hist[lane * PER + i];
+            if (run + c >= krem) {
+                b = static_cast<uint32_t>(lane * PER + i);
+                r = krem - run;
+                break;
             }
-            __syncthreads();
+            run += c;
         }
     }
-    if (threadIdx.x == 0) ctrl->thr = fminf((k <= n) ? a[k - 1] : kInf, cap);
+    bin = __shfl(b, L, kWave);
+    rank_in = __shfl(r, L, kWave);
+}
+
+// k-th smallest of n (<= kSeedMax) values, exactly: radix selection over the
+// order-preserving keys, 8 bits per pass from the top, with the n values held
+// in registers (kSeedMax / 1024 per thread) and a 256-bin LDS histogram of
+// the values that still match the selected prefix.
+constexpr int kSeedMax = 16384;
+constexpr int kSeedThreads = 1024;
+constexpr int kSeedPer = kSeedMax / kSeedThreads;
+
+__global__ __launch_bounds__(kSeedThreads) void seed_thr_kernel(const float* __restrict__ v,
+                                                                int n, int k, float cap,
+                                                                Ctrl* ctrl) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sel[2];
+    if (k > n || k < 1) {
+        if (threadIdx.x == 0) ctrl->thr = fminf(kInf, cap);
+        return;
+    }
+    uint32_t key[kSeedPer];
+#pragma unroll
+    for (int i = 0; i < kSeedPer; ++i) {
+        const int j = static_cast<int>(threadIdx.x) + i * kSeedThreads;
+        key[i] = j < n ? ord_key(v[j]) : 0u;
+    }
+    uint32_t prefix = 0, krem = static_cast<uint32_t>(k);
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        if (threadIdx.x < 256) hist[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t hmask = shift == 24 ? 0u : (0xffffffffu << (shift + 8));
+#pragma unroll
+        for (int i = 0; i < kSeedPer; ++i) {
+            const int j = static_cast<int>(threadIdx.x) + i * kSeedThreads;
+            if (j < n && (key[i] & hmask) == prefix)
+                atomicAdd(&hist[(key[i] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < kWave) {
+            uint32_t bin, r;
+            wave_find_rank<4>(hist, krem, bin, r);
+            if (threadIdx.x == 0) {
+                sel[0] = prefix | (bin << shift);  // a bin exists: krem <= matching count
+                sel[1] = r;
+            }
+        }
+        __syncthreads();
+        prefix = sel[0];
+        krem = sel[1];
+    }
+    if (threadIdx.x == 0) ctrl->thr = fminf(unord_key(prefix), cap);
 }
 
 inline int check_launch() {
@@ -202,54 +272,31 @@ __global__ __launch_bounds__(256) void tighten_hist_kernel(const uint32_t* __res
         if (h[i]) atomicAdd(&ws[i], h[i]);
 }
 
-// One block: find the bin of the remaining rank per parity, clear the
-// histogram for the next pass; after pass 1 lower ctrl->thr.
+// One block: find the bin of the remaining rank per parity (one wave per
+// parity), clear the histogram for the next pass; after pass 1 lower
+// ctrl->thr.
 __global__ __launch_bounds__(256) void tighten_find_kernel(uint32_t* __restrict__ ws, int32_t k,
                                                            int split, int pass,
                                                            Ctrl* __restrict__ ctrl) {
-    __shared__ uint32_t part[256];
-    __shared__ uint32_t found[2];
+    __shared__ uint32_t found[2], rank[2];
     const int t = threadIdx.x;
-    constexpr int per = kSelBins / 256;
-    for (int par = 0; par < (split ? 2 : 1); ++par) {
-        uint32_t* h = ws + par * kSelBins;
-        const uint32_t krem = pass == 0 ? static_cast<uint32_t>(k) : ws[kSelState + 2 + par];
-        uint32_t sum = 0;
-        for (int i = 0; i < per; ++i) sum += h[t * per + i];
-        part[t] = sum;
-        __syncthreads();
-        if (t == 0) {
-            // Running count over blocks of bins, then inside the block.
-            uint32_t run = 0;
-            int blk = -1;
-            for (int i = 0; i < 256; ++i) {
-                if (run + part[i] >= krem) {
-                    blk = i;
-                    break;
-                }
-                run += part[i];
-            }
-            uint32_t bin = 0xffffffffu;
-            if (blk >= 0) {
-                for (int i = 0; i < per; ++i) {
-                    const uint32_t c = h[blk * per + i];
-                    if (run + c >= krem) {
-                        bin = static_cast<uint32_t>(blk * per + i);
-                        break;
-                    }
-                    run += c;
-                }
-            }
-            found[par] = bin;
-            ws[kSelState + 2 + par] = krem - run;  // rank inside the bin
+    const int wv = t / kWave;
+    if (wv < (split ? 2 : 1)) {
+        const uint32_t krem = pass == 0 ? static_cast<uint32_t>(k) : ws[kSelState + 2 + wv];
+        uint32_t bin, r;
+        wave_find_rank<kSelBins / kWave>(ws + wv * kSelBins, krem, bin, r);
+        if ((t & (kWave - 1)) == 0) {
+            found[wv] = bin;
+            rank[wv] = r;
         }
-        __syncthreads();
     }
+    __syncthreads();
     for (int i = t; i < 2 * kSelBins; i += blockDim.x) ws[i] = 0;
     if (t != 0) return;
     uint32_t bound[2] = {0x7f800000u, 0x7f800000u};
     for (int par = 0; par < (split ? 2 : 1); ++par) {
-        if (found[par] == 0xffffffffu) {  // fewer than k known: no bound
+        ws[kSelState + 2 + par] = rank[par];  // rank inside the bin
+        if (found[par] == 0xffffffffu) {      // fewer than k known: no bound
             ws[kSelState + par] = 0xffffffffu;
             continue;
         }
@@ -266,6 +313,70 @@ __global__ __launch_bounds__(256) void tighten_find_kernel(uint32_t* __restrict_
         const float tb = __uint_as_float(b);
         if (tb < ctrl->thr) ctrl->thr = tb;
     }
+}
+
+// Both passes in one block, histograms in LDS, for up to kTightenSmall
+// buckets (where the four launches of the general path cost more than one
+// block reading the minima twice).  Same bins, same bound.
+constexpr int64_t kTightenSmall = 1 << 15;
+constexpr int kTightenThreads = 1024;
+constexpr int kTightenUnroll = 8;
+
+__global__ __launch_bounds__(kTightenThreads) void tighten_small_kernel(
+    const uint32_t* __restrict__ bmin, int64_t nb, int32_t k, int split, Ctrl* __restrict__ ctrl) {
+    __shared__ uint32_t h[2 * kSelBins];
+    __shared__ uint32_t prefix[2], rank[2], sub[2];
+    const int t = threadIdx.x;
+    const int wv = t / kWave;
+    const int npar = split ? 2 : 1;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int i = t; i < 2 * kSelBins; i += kTightenThreads) h[i] = 0;
+        __syncthreads();
+        const uint32_t p0 = pass ? prefix[0] : 0u, p1 = pass ? prefix[1] : 0u;
+        for (int64_t b0 = t; b0 < nb; b0 += static_cast<int64_t>(kTightenThreads) * kTightenUnroll) {
+            uint32_t v[kTightenUnroll];
+#pragma unroll
+            for (int u = 0; u < kTightenUnroll; ++u) {
+                const int64_t b = b0 + static_cast<int64_t>(u) * kTightenThreads;
+                v[u] = b < nb ? bmin[b] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kTightenUnroll; ++u) {
+                const int64_t b = b0 + static_cast<int64_t>(u) * kTightenThreads;
+                if (b >= nb) break;
+                const int par = split ? static_cast<int>(b & 1) : 0;
+                if (pass == 0)
+                    atomicAdd(&h[par * kSelBins + (v[u] >> 20)], 1u);
+                else if ((v[u] >> 20) == (par ? p1 : p0))
+                    atomicAdd(&h[par * kSelBins + ((v[u] >> 8) & 0xfffu)], 1u);
+            }
+        }
+        __syncthreads();
+        if (wv < npar) {
+            const uint32_t krem = pass == 0 ? static_cast<uint32_t>(k) : rank[wv];
+            uint32_t bin, r;
+            wave_find_rank<kSelBins / kWave>(h + wv * kSelBins, krem, bin, r);
+            if ((t & (kWave - 1)) == 0) {
+                if (pass == 0) {
+                    prefix[wv] = bin;  // 0xffffffff: fewer than k buckets known
+                    rank[wv] = r;
+                } else {
+                    sub[wv] = bin;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    uint32_t b = 0x7f800000u;
+    for (int par = 0; par < npar; ++par) {
+        if (prefix[par] == 0xffffffffu || sub[par] == 0xffffffffu) continue;
+        uint32_t hi = (prefix[par] << 20) | (sub[par] << 8) | 0xffu;
+        if (hi > 0x7f800000u) hi = 0x7f800000u;
+        if (hi < b) b = hi;
+    }
+    const float tb = __uint_as_float(b);
+    if (tb < ctrl->thr) ctrl->thr = tb;
 }
 
 // ---- greedy top-k selection on the device ---------------------------------
@@ -400,6 +511,11 @@ size_t tighten_ws_words() { return kSelState + 4; }
 int launch_tighten(hipStream_t st, const uint32_t* bmin, int64_t nb, int32_t k, bool split,
                    uint32_t* scratch, Ctrl* ctrl) {
     if (nb <= 0) return MDTW_OK;
+    if (nb <= kTightenSmall) {
+        hipLaunchKernelGGL(tighten_small_kernel, dim3(1), dim3(kTightenThreads), 0, st, bmin, nb,
+                           k, split ? 1 : 0, ctrl);
+        return check_launch();
+    }
     const int64_t blocks = std::min<int64_t>((nb + 255) / 256, 1024);
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(tighten_hist_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
