@@ -40,11 +40,11 @@ constexpr int kBandWaves = 4;  // waves per workgroup
 // (up to W = 23; measured over W = 0..112 against the chunked loads, which
 // stall on their scalar waits for W = 18..23 and are faster from W = 24).
 constexpr int kBandSlideMax = 54;
-// Bands up to kBandMaxW unroll kBandU columns per block (W = 108..111: two,
-// W = 112: one, which keeps them within 256 registers, i.e. two waves per
-// SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which keeps
-// the 2W + 1 band values in vector + accumulation registers (no scratch
-// memory) at one wave per SIMD.
+// Bands up to kBandMaxW unroll kBandU columns per block (W = 108..110: two,
+// W = 111..112: one, which keeps them within 256 registers, i.e. two waves
+// per SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which
+// keeps the 2W + 1 band values in vector + accumulation registers (no scratch
+// memory): two waves per SIMD up to W = 114, one from W = 115.
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 
@@ -55,22 +55,25 @@ __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane) {
     return static_cast<int64_t>(__shfl(b, 0, kWave));
 }
 
-// Next 64 windows from the shared cursor.  Once a wave of the block has seen
-// the cursor past the end (it only grows), the block's other waves stop
-// without another same-address atomic: at the end of a launch every wave
-// would otherwise queue one on the cursor at once.
-__device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t count, int* drained) {
-    unsigned long long b = 0;
-    if (lane == 0) {
-        if (__hip_atomic_load(drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-            b = static_cast<unsigned long long>(count);
-        } else {
-            b = atomicAdd(&ctrl->work, static_cast<unsigned long long>(kWave));
-            if (b >= static_cast<unsigned long long>(count))
-                __hip_atomic_store(drained, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
+// Next 64 windows from the shared cursor, which hands out the windows from
+// `first` on (the ones before were assigned by grid position).  Once a wave
+// of the block has seen the cursor past the end (it only grows), the block's
+// other waves stop without another same-address atomic: at the end of a
+// launch every wave would otherwise queue one on the cursor at once.
+__device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t first, int64_t count,
+                                             int* drained) {
+    unsigned long long b = static_cast<unsigned long long>(count);
+    if (lane == 0 && first < count &&
+        !__hip_atomic_load(drained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+        b = static_cast<unsigned long long>(first) +
+            atomicAdd(&ctrl->work, static_cast<unsigned long long>(kWave));
+        if (b >= static_cast<unsigned long long>(count))
+            __hip_atomic_store(drained, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    return static_cast<int64_t>(__shfl(b, 0, kWave));
+    // lane 0's value, kept in scalar registers (the whole wave calls this)
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(b >> 32));
+    return static_cast<int64_t>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
 // z = (x - mu) / sigma, bit-identical to the IEEE quotient: q = a * y with y
@@ -112,7 +115,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                                          const float* __restrict__ elo,
                                          const float* __restrict__ eup, float lbt) {
     constexpr int B = 2 * W + 1;
-    constexpr int U = W > 111 ? 1 : (W > 107 ? 2 : kBandU);
+    constexpr int U = W > 110 ? 1 : (W > 107 ? 2 : kBandU);
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
     // Narrow bands keep the query window of a whole block of U columns in
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(256) void band_kernel(
     const float* __restrict__ lbt, const float* __restrict__ elo,
     const float* __restrict__ eup) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wid = threadIdx.x / kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     int64_t count = a.count;
     if (a.idx_count_from_ctrl)
         count = static_cast<int64_t>(
@@ -324,10 +327,7 @@ __global__ __launch_bounds__(256) void band_kernel(
         }
         n_dp += __popcll(__ballot(valid));
         n_ab += __popcll(__ballot(valid && !alive));
-        if (!a.persistent)
-            base += step;
-        else
-            base = step >= count ? count : step + band_next(ctrl, lane, count - step, &drained);
+        base = a.persistent ? band_next(ctrl, lane, step, count, &drained) : base + step;
     }
     // Counters summed per block: two atomics per block, not per wave.
     if (a.mode != kModeProfile) {
