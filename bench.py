@@ -93,7 +93,7 @@ def timed(torch, fn, reps=1):
 
 
 def section_c2(eng, torch, args):
-    from cudadtw_amd import ALGO_ED, synth, znormalize
+    from cudadtw_amd import ALGO_ED, ALGO_ED_FFT, synth, znormalize
 
     m, w = 256, 26
     log(f"[c2] generating ECG-like stream 2^{args.log2n} ...")
@@ -154,6 +154,24 @@ def section_c2(eng, torch, args):
     re_, t_ed = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=5)
     log(f"[c2] ED search: {t_ed*1e3:.2f} ms, location {re_.location}")
 
+    # ED through FP64 FFT cross-correlation (the reference's form, ed.cuh:71-153)
+    # against the direct ED search, for the C2 query and two long queries
+    # cut from the stream (|Q| = 2048, 8192; the FFT path's limit is 8192).
+    ed_forms = {}
+    for mq in (m, 2048, 8192):
+        zz = zq if mq == m else znormalize(synth.cut_query(stream, mq, 7 + mq)[0])
+        eng.set_query(zz, 0)
+        rd, td = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=3)
+        rf, tf = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED_FFT), reps=3)
+        Lq = n - mq + 1
+        ed_forms[str(mq)] = {"direct_ms": round(td * 1e3, 3), "fft_ms": round(tf * 1e3, 3),
+                             "fft_windows_per_s": round(Lq / tf, 1),
+                             "direct_location": rd.location, "fft_location": rf.location,
+                             "same_location": rd.location == rf.location}
+        log(f"[c2] ED |Q|={mq}: direct {td*1e3:.2f} ms (location {rd.location}), FFT "
+            f"{tf*1e3:.2f} ms (location {rf.location})")
+    eng.set_query(zq, w)
+
     # The same search fed from host memory: one-shot after a host-to-device
     # copy, and the streaming search (2^24-window epochs, pushes of 2^24
     # samples, copies overlapped with the search).  PCIe included in both.
@@ -189,6 +207,7 @@ def section_c2(eng, torch, args):
                           "lb_keogh_pruned": r.info["lb_keogh_pruned"]},
         "ed_search": {"ms": round(t_ed * 1e3, 3), "windows_per_s": round(L / t_ed, 1),
                       "location": re_.location},
+        "ed_direct_vs_fft": ed_forms,
         "from_host": {"copy_then_search_ms": round(t_copy * 1e3, 2),
                       "streamed_ms": round(t_stream * 1e3, 2),
                       "streamed_windows_per_s": round(L / t_stream, 1),
