@@ -18,6 +18,8 @@ Also on the same line, one object per workload:
   c4      mixed batch: queries |Q| in [64, 1024], bands 5-20 %, top-8, one
           captured-graph batch launch vs the same queries as one plain search
           each (default: every 32nd of the 1024 queries; --c4 full: all),
+          plus 1024 short queries on a 2^18 stream, where per-query fixed
+          costs dominate (--c4 short: only that case),
   c5      256 queries |Q| in [128, 512], W = 10 %, 2^24 stream: top-8 forward,
           query-gradient backward (recompute GCUPS), 50-step template
           refinement,
@@ -27,7 +29,7 @@ reference CPU path, brute-force banded DTW over all windows with OpenMP,
 cdtw.cuh:90-157, on a bounded sample; and the single-threaded UCR-cascade
 port).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--c4 sample|full|off]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--c4 sample|full|short|off]
 """
 import argparse
 import json
@@ -256,7 +258,43 @@ def section_c3(eng, torch):
     return out
 
 
+def c4_short(eng, torch, k=8):
+    """The batch-vs-plain comparison where per-query fixed costs dominate:
+    1024 short queries on a short stream (launch- and synchronisation-bound),
+    to show what one graph launch over concurrent pipelines removes."""
+    from cudadtw_amd import synth, znormalize
+
+    s2 = synth.ecg_like(1 << 18, 9)
+    q2, b2, _ = synth.batch_queries(s2, 1024, 9, 64, 128)
+    z2 = [znormalize(q) for q in q2]
+    ds2 = torch.from_numpy(s2).to("cuda:0")
+    bat2 = eng.batch(z2, b2, pipelines=4)
+    bat2.search(ds2, k=k)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r2 = bat2.search(ds2, k=k)
+    t_b2 = time.perf_counter() - t0
+    p2 = np.full((len(z2), k), -1, np.int64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (z, w) in enumerate(zip(z2, b2)):
+        eng.set_query(z, int(w))
+        p2[i] = eng.search(ds2, int(w), k=k, excl=z.size // 2).positions
+    torch.cuda.synchronize()
+    t_p2 = time.perf_counter() - t0
+    same2 = bool(np.array_equal(p2, r2.positions))
+    bat2.close()
+    log(f"[c4] short queries: batch {t_b2*1e3:.1f} ms vs plain {t_p2*1e3:.1f} ms "
+        f"({t_p2 / t_b2:.2f}x), identical: {same2}")
+    return {"workload": "ECG-like 2^18, 1024 queries |Q| 64-128, bands 5-20%, top-8 "
+                        "(not the C4 shape)",
+            "batch_ms": round(t_b2 * 1e3, 2), "plain_ms": round(t_p2 * 1e3, 2),
+            "speedup": round(t_p2 / t_b2, 3), "identical": same2}
+
+
 def section_c4(eng, torch, args):
+    if args.c4 == "short":
+        return {"launch_bound_case": c4_short(eng, torch)}
     from cudadtw_amd import synth, znormalize
 
     log("[c4] generating ECG-like stream 2^26 and 1024 queries ...")
@@ -294,31 +332,7 @@ def section_c4(eng, torch, args):
         f"identical results: {same}")
     batch.close()
 
-    # The same comparison where per-query fixed costs dominate: 1024 short
-    # queries on a short stream (launch- and synchronisation-bound), to show
-    # what one graph launch over concurrent pipelines removes.
-    s2 = synth.ecg_like(1 << 18, 9)
-    q2, b2, _ = synth.batch_queries(s2, 1024, 9, 64, 128)
-    z2 = [znormalize(q) for q in q2]
-    ds2 = torch.from_numpy(s2).to("cuda:0")
-    bat2 = eng.batch(z2, b2, pipelines=4)
-    bat2.search(ds2, k=k)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    r2 = bat2.search(ds2, k=k)
-    t_b2 = time.perf_counter() - t0
-    p2 = np.full((len(z2), k), -1, np.int64)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i, (z, w) in enumerate(zip(z2, b2)):
-        eng.set_query(z, int(w))
-        p2[i] = eng.search(ds2, int(w), k=k, excl=z.size // 2).positions
-    torch.cuda.synchronize()
-    t_p2 = time.perf_counter() - t0
-    same2 = bool(np.array_equal(p2, r2.positions))
-    bat2.close()
-    log(f"[c4] short queries: batch {t_b2*1e3:.1f} ms vs plain {t_p2*1e3:.1f} ms "
-        f"({t_p2 / t_b2:.2f}x), identical: {same2}")
+    short = c4_short(eng, torch, k)
     return {"workload": f"ECG-like 2^26, {len(zqs)} of 1024 queries "
                         f"({'all' if args.c4 == 'full' else 'every %d-th' % args.c4_stride}), "
                         "|Q| 64-1024, bands 5-20%, half cut+N(0,0.05), half walks, top-8, "
@@ -327,11 +341,7 @@ def section_c4(eng, torch, args):
             "batch_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
             "speedup": round(t_plain / t_batch, 3), "identical": same,
             "tune_s": round(t_tune, 2), "buckets_tuned": nb,
-            "launch_bound_case": {"workload": "ECG-like 2^18, 1024 queries |Q| 64-128, "
-                                              "bands 5-20%, top-8 (not the C4 shape)",
-                                  "batch_ms": round(t_b2 * 1e3, 2),
-                                  "plain_ms": round(t_p2 * 1e3, 2),
-                                  "speedup": round(t_p2 / t_b2, 3), "identical": same2},
+            "launch_bound_case": short,
             "windows_per_s_batch": round(float(Ls.sum()) / t_batch, 1),
             "effective_gcups_batch": round(cellsum / t_batch / 1e9, 2)}
 
@@ -433,7 +443,7 @@ def main():
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="c2,c1,c3,c4,c5")
-    ap.add_argument("--c4", choices=["sample", "full", "off"], default="sample")
+    ap.add_argument("--c4", choices=["sample", "full", "short", "off"], default="sample")
     ap.add_argument("--c4-stride", type=int, default=32)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (HBM counter calibration)")

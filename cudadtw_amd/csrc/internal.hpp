@@ -325,6 +325,11 @@ int launch_ed(hipStream_t st, int cus, const BandArgs& a);
 constexpr unsigned long long kBestNone = (0x7f800000ull << 32) | 0xffffffffull;
 int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init,
                       uint32_t* bmin = nullptr, int64_t bucket_len = 1);
+// A search's start in one launch: the control block (as launch_ctrl_reset),
+// the nb bucket minima to +inf bits and the ntw tighten-scratch words to 0.
+int launch_search_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init,
+                        uint32_t* bmin, int64_t bucket_len, int64_t nb, uint32_t* tscratch,
+                        int64_t ntw);
 // Between search rounds: candidate count and work cursor back to 0.
 int launch_round_reset(hipStream_t st, Ctrl* ctrl);
 // Top-k bound from the bucket minima (nb buckets, +inf where unknown):
@@ -334,10 +339,11 @@ int launch_round_reset(hipStream_t st, Ctrl* ctrl);
 // more than 2*excl apart, so one greedy pick (which excludes |i - j| < excl)
 // removes at most one of them; the k-th pick is then no worse than the k-th
 // smallest of either parity class.  scratch: tighten_ws_words() words.
+// reset_round: also set the candidate count and work cursor back to 0 at the
+// end (the next round's start, as launch_round_reset).
 size_t tighten_ws_words();
 int launch_tighten(hipStream_t st, const uint32_t* bmin, int64_t nb, int32_t k, bool split,
-                   uint32_t* scratch, Ctrl* ctrl);
-int launch_fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v);
+                   uint32_t* scratch, Ctrl* ctrl, bool reset_round = false);
 int launch_thr_from_best(hipStream_t st, Ctrl* ctrl);
 int launch_argmin(hipStream_t st, const float* v, int64_t count, Ctrl* ctrl);
 size_t threshold_ws_ints(int64_t count);

@@ -215,10 +215,8 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
 
     hipStream_t st = p.st;
     const unsigned long long best0 = p.best_init < kBestNone ? p.best_init : kBestNone;
-    int rc = launch_ctrl_reset(st, p.ctrl, best0, bmin, tl.bucket_len);
-    if (rc == MDTW_OK && topk) rc = launch_fill_u32(st, bmin, tl.nb, 0x7f800000u);
-    if (rc == MDTW_OK && topk)
-        rc = launch_fill_u32(st, tscratch, static_cast<int64_t>(tighten_ws_words()), 0u);
+    int rc = launch_search_reset(st, p.ctrl, best0, bmin, tl.bucket_len, topk ? tl.nb : 0,
+                                 tscratch, topk ? static_cast<int64_t>(tighten_ws_words()) : 0);
     if (rc == MDTW_OK && own_stats) {
         rc = p.stats_mode == MDTW_STATS_REFCOMPAT ? launch_stats_refcompat(p.e, p.s, p.n, m, avg, sd)
              : p.stats_mode == MDTW_STATS_NONE  ? launch_stats_none(st, p.n - m + 1, avg, sd)
@@ -255,7 +253,8 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     // 3. rounds: the bound left by earlier rounds (k = 1: the best distance
     // so far; k > 1: the bucket bound) prunes the later ones.
     const int64_t cascade_cap = static_cast<int64_t>(p.cus) * p.shape.cascade_blocks_per_cu;
-    for (const Round& r : rounds) {
+    for (size_t ri = 0; ri < rounds.size(); ++ri) {
+        const Round& r = rounds[ri];
         if (p.algo == MDTW_ALGO_ED) {
             // ED is cheap per window and its own early abandon is the filter.
             BandArgs fa = a;
@@ -264,11 +263,10 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
             fa.mode = topk ? kModeCollect : kModeBest;
             rc = run_algo(p, fa);
         } else if (m >= 11) {
-            rc = launch_round_reset(st, p.ctrl);
-            if (rc == MDTW_OK)
-                rc = launch_cascade(st, p.s, avg, sd, p.q, p.lo, p.up, m, r.base, r.count,
-                                    p.metric, !topk, p.ctrl, cand, cand_lb, cascade_cap,
-                                    p.cus);
+            // The candidate count and work cursor are 0 here: the search
+            // reset cleared them, and between rounds the tighten does.
+            rc = launch_cascade(st, p.s, avg, sd, p.q, p.lo, p.up, m, r.base, r.count, p.metric,
+                                !topk, p.ctrl, cand, cand_lb, cascade_cap, p.cus);
             if (rc == MDTW_OK) {
                 BandArgs da = a;
                 da.idx = cand;
@@ -289,7 +287,8 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
             rc = run_algo(p, da);
         }
         if (rc == MDTW_OK && topk)
-            rc = launch_tighten(st, bmin, tl.nb, k, tl.split, tscratch, p.ctrl);
+            rc = launch_tighten(st, bmin, tl.nb, k, tl.split, tscratch, p.ctrl,
+                                ri + 1 < rounds.size());
         if (rc != MDTW_OK) return rc;
     }
     return MDTW_OK;

@@ -25,6 +25,30 @@ __global__ void ctrl_reset_kernel(Ctrl* ctrl, unsigned long long best_init, uint
     }
 }
 
+__global__ __launch_bounds__(256) void search_reset_kernel(Ctrl* ctrl,
+                                                           unsigned long long best_init,
+                                                           uint32_t* bmin, long long bucket_len,
+                                                           int64_t nb, uint32_t* tscratch,
+                                                           int64_t ntw) {
+    const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    if (g == 0) {
+        ctrl->bmin = bmin;
+        ctrl->bucket_len = bucket_len;
+        ctrl->best = best_init;
+        ctrl->count = 0;
+        ctrl->work = 0;
+        ctrl->nres = 0;
+        ctrl->kim = 0;
+        ctrl->keogh = 0;
+        ctrl->dp = 0;
+        ctrl->abandoned = 0;
+        ctrl->thr = kInf;
+    }
+    for (int64_t i = g; i < nb; i += stride) bmin[i] = 0x7f800000u;
+    for (int64_t i = g; i < ntw; i += stride) tscratch[i] = 0u;
+}
+
 __global__ void ctrl_set_thr_from_best(Ctrl* ctrl) {
     if (threadIdx.x == 0) ctrl->thr = key_dist(ctrl->best);
 }
@@ -232,13 +256,6 @@ __global__ void round_reset_kernel(Ctrl* ctrl) {
     }
 }
 
-__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += stride)
-        p[i] = v;
-}
-
 // ---- top-k bound: two-pass radix selection over the bucket minima ----------
 // Pass 0 histograms bits 31..20 of every bucket minimum (non-negative float
 // bits order like the values), per parity class; the k-th smallest lies in
@@ -277,7 +294,8 @@ __global__ __launch_bounds__(256) void tighten_hist_kernel(const uint32_t* __res
 // ctrl->thr.
 __global__ __launch_bounds__(256) void tighten_find_kernel(uint32_t* __restrict__ ws, int32_t k,
                                                            int split, int pass,
-                                                           Ctrl* __restrict__ ctrl) {
+                                                           Ctrl* __restrict__ ctrl,
+                                                           int reset_round) {
     __shared__ uint32_t found[2], rank[2];
     const int t = threadIdx.x;
     const int wv = t / kWave;
@@ -312,6 +330,10 @@ __global__ __launch_bounds__(256) void tighten_find_kernel(uint32_t* __restrict_
         if (split && bound[1] < b) b = bound[1];
         const float tb = __uint_as_float(b);
         if (tb < ctrl->thr) ctrl->thr = tb;
+        if (reset_round) {
+            ctrl->count = 0;
+            ctrl->work = 0;
+        }
     }
 }
 
@@ -323,7 +345,8 @@ constexpr int kTightenThreads = 1024;
 constexpr int kTightenUnroll = 8;
 
 __global__ __launch_bounds__(kTightenThreads) void tighten_small_kernel(
-    const uint32_t* __restrict__ bmin, int64_t nb, int32_t k, int split, Ctrl* __restrict__ ctrl) {
+    const uint32_t* __restrict__ bmin, int64_t nb, int32_t k, int split, Ctrl* __restrict__ ctrl,
+    int reset_round) {
     __shared__ uint32_t h[2 * kSelBins];
     __shared__ uint32_t prefix[2], rank[2], sub[2];
     const int t = threadIdx.x;
@@ -377,6 +400,10 @@ __global__ __launch_bounds__(kTightenThreads) void tighten_small_kernel(
     }
     const float tb = __uint_as_float(b);
     if (tb < ctrl->thr) ctrl->thr = tb;
+    if (reset_round) {
+        ctrl->count = 0;
+        ctrl->work = 0;
+    }
 }
 
 // ---- greedy top-k selection on the device ---------------------------------
@@ -491,29 +518,32 @@ int launch_ctrl_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init, 
     return check_launch();
 }
 
+int launch_search_reset(hipStream_t st, Ctrl* ctrl, unsigned long long best_init,
+                        uint32_t* bmin, int64_t bucket_len, int64_t nb, uint32_t* tscratch,
+                        int64_t ntw) {
+    const int64_t n = std::max<int64_t>(nb, ntw);
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(search_reset_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                       ctrl, best_init, bmin, static_cast<long long>(bucket_len), nb, tscratch,
+                       ntw);
+    return check_launch();
+}
+
 int launch_round_reset(hipStream_t st, Ctrl* ctrl) {
     hipLaunchKernelGGL(round_reset_kernel, dim3(1), dim3(64), 0, st, ctrl);
     return check_launch();
 }
 
-int launch_fill_u32(hipStream_t st, uint32_t* p, int64_t n, uint32_t v) {
-    if (n <= 0) return MDTW_OK;
-    const int64_t nb = std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, st, p, n,
-                       v);
-    return check_launch();
-}
-
 size_t tighten_ws_words() { return kSelState + 4; }
 
-// scratch must hold zeros in its histogram part on entry (launch_fill_u32
-// once); every pass leaves it cleared again.
+// scratch must hold zeros in its histogram part on entry (launch_search_reset
+// clears it); every pass leaves it cleared again.
 int launch_tighten(hipStream_t st, const uint32_t* bmin, int64_t nb, int32_t k, bool split,
-                   uint32_t* scratch, Ctrl* ctrl) {
-    if (nb <= 0) return MDTW_OK;
+                   uint32_t* scratch, Ctrl* ctrl, bool reset_round) {
+    if (nb <= 0) return reset_round ? launch_round_reset(st, ctrl) : MDTW_OK;
     if (nb <= kTightenSmall) {
         hipLaunchKernelGGL(tighten_small_kernel, dim3(1), dim3(kTightenThreads), 0, st, bmin, nb,
-                           k, split ? 1 : 0, ctrl);
+                           k, split ? 1 : 0, ctrl, reset_round ? 1 : 0);
         return check_launch();
     }
     const int64_t blocks = std::min<int64_t>((nb + 255) / 256, 1024);
@@ -521,7 +551,7 @@ int launch_tighten(hipStream_t st, const uint32_t* bmin, int64_t nb, int32_t k, 
         hipLaunchKernelGGL(tighten_hist_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
                            st, bmin, nb, split ? 1 : 0, pass, scratch);
         hipLaunchKernelGGL(tighten_find_kernel, dim3(1), dim3(256), 0, st, scratch, k,
-                           split ? 1 : 0, pass, ctrl);
+                           split ? 1 : 0, pass, ctrl, reset_round && pass == 1 ? 1 : 0);
     }
     return check_launch();
 }
