@@ -83,6 +83,7 @@ __device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
     int ib = -kWave;
     float qb = kInf;
     const bool abandon = mode == kModeBest || mode == kModeCollect;
+    const float thr = abandon ? load_abandon_bound(ctrl, mode) : kInf;
     const int K = 2 * m - 1;
     for (int k = 0; k < K; ++k) {
         const int par = (k + W) & 1;
@@ -149,7 +150,6 @@ __device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
 #pragma unroll
             for (int c = 0; c < C; ++c) cm = fminf(cm, fminf(A1[c], A2[c]));
             cm = wave_min_f32(cm);
-            const float thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
             if (cm > thr) {
                 alive = false;
                 return kInf;

@@ -44,7 +44,7 @@ constexpr int kBandSlideMax = 54;
 // W = 111..112: one, which keeps them within 256 registers, i.e. two waves
 // per SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which
 // keeps the 2W + 1 band values in vector + accumulation registers (no scratch
-// memory): two waves per SIMD up to W = 114, one from W = 115.
+// memory): two waves per SIMD up to W = 113, one from W = 114.
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 
@@ -148,11 +148,11 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
 
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     const int lane = threadIdx.x & (kWave - 1);
+    const float thr = abandon ? load_abandon_bound(ctrl, mode) : kInf;
     float lo_c = 0.0f, up_c = 0.0f, lo_n = 0.0f, up_n = 0.0f, P = 0.0f;
     if constexpr (LB) {
         // The bound may have fallen since the cascade kept this window.
-        const float thr0 = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
-        alive = alive && !(lbt > prune_limit(thr0, m, L2));
+        alive = alive && !(lbt > prune_limit(thr, m, L2));
         if (!__any(alive)) return kInf;
         lo_c = elo[lane < m ? lane : m - 1];
         up_c = eup[lane < m ? lane : m - 1];
@@ -245,7 +245,6 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
             float cm = E[0];
 #pragma unroll
             for (int t = 1; t < B; ++t) cm = fminf(cm, E[t]);
-            const float thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
             if constexpr (LB) {
                 const float rest = lbt - P;
                 const float bound = cm + (rest > 0.0f ? rest : 0.0f);

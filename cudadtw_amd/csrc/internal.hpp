@@ -384,6 +384,17 @@ __device__ __forceinline__ float load_best_dist(const Ctrl* c) {
 __device__ __forceinline__ float load_thr(const Ctrl* c) {
     return __hip_atomic_load(&c->thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The bound a DP abandons against (mode kModeBest: the best distance so far;
+// kModeCollect: the top-k threshold), read once per batch of windows and kept
+// in a scalar register.  Both only fall while a launch runs (atomicMin on the
+// best; the top-k bound changes between launches), so a value read earlier
+// is a valid, slightly looser bound.  Re-reading it at every abandon check
+// sent every wave of the chip to the same L2 line every few columns, where
+// the reads serialise: the search DP spent most of its cycles waiting.
+__device__ __forceinline__ float load_abandon_bound(const Ctrl* c, int mode) {
+    const float v = mode == kModeBest ? load_best_dist(c) : load_thr(c);
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
 
 // Exact distance d of window w is known: lower its bucket's minimum.
 __device__ __forceinline__ void note_bucket(uint32_t* bmin, long long bucket_len, int64_t w,

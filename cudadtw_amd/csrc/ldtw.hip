@@ -40,6 +40,7 @@ __device__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
     float zj1 = M > 1 ? (x[1] - mu) / sg : 0.0f;
     float d = lcost<L2>(qi - zj);
     const bool abandon = mode == kModeBest || mode == kModeCollect;
+    const float thr = abandon ? load_abandon_bound(ctrl, mode) : kInf;
     int step = 0;
     // A valid greedy path reaches (M-1, M-1) in at most 2M-2 moves (every
     // move advances i or j and the path never leaves the band); the cap and
@@ -80,7 +81,6 @@ __device__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
         }
         ++step;
         if (abandon && (step % kLdtwCheck) == 0) {
-            const float thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
             alive = alive && !(d > thr);
             if (!__any(alive)) return kInf;
         }
