@@ -156,6 +156,26 @@ def section_c2(eng, torch, args):
     re_, t_ed = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=5)
     log(f"[c2] ED search: {t_ed*1e3:.2f} ms, location {re_.location}")
 
+    # DP rate on wide bands (|Q| = 1024 at 10 / 15.6 / 20 %: W = 102 in the
+    # four-column register kernel, 160 and 205 in the one-wave-per-SIMD wide
+    # register kernel), profile mode over a 2^24-sample prefix of the stream.
+    wide = {}
+    n24 = min(n, 1 << 24)
+    q1k = znormalize(synth.cut_query(stream[:n24], 1024, 11)[0])
+    for wb in (102, 160, 205):
+        eng.set_query(q1k, wb)
+        a1, s1 = eng.avg_std(ds[:n24], 1024)
+        p1 = eng.gpu_cdtw(ds[:n24], a1, s1, wb)
+        st.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        eng.gpu_cdtw(ds[:n24], a1, s1, wb, out=p1)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        wide[str(wb)] = round((n24 - 1023) * cells(1024, wb) / (ms / 1e3) / 1e9, 1)
+    log(f"[c2] wide-band DP GCUPS (|Q| = 1024): {wide}")
+
     # ED through FP64 FFT cross-correlation (the reference's form, ed.cuh:71-153)
     # against the direct ED search, for the C2 query and two long queries
     # cut from the stream (|Q| = 2048, 8192; the FFT path's limit is 8192).
@@ -201,6 +221,7 @@ def section_c2(eng, torch, args):
         "main": results[w], "L": L, "n": n, "m": m, "w": w,
         "gcups_by_band": {str(k): round(v["gcups"], 2) for k, v in results.items()},
         "dp_kernel_gcups_by_band": {str(k): round(v["dp_gcups"], 2) for k, v in results.items()},
+        "dp_kernel_gcups_wide_m1024": wide,
         "pruned_search": {"ms": round(t_search * 1e3, 3), "windows_per_s": round(L / t_search, 1),
                           "effective_gcups": round(L * cells(m, w) / t_search / 1e9, 2),
                           "location": r.location, "distance": round(r.distance, 6),

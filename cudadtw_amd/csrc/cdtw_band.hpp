@@ -148,11 +148,21 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
 
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     const int lane = threadIdx.x & (kWave - 1);
-    const float thr = abandon ? load_abandon_bound(ctrl, mode) : kInf;
+    // Bands that fit in vector registers read the abandon bound once per
+    // batch of windows (load_abandon_bound); the wide ones re-read it at each
+    // check: their band already fills the accumulation registers, one more
+    // value live across the DP made the compiler spill it to scratch memory,
+    // and their checks are hundreds of cells apart anyway.
+    constexpr bool kBoundOnce = W <= kBandMaxW;
+    const float thr_once = kBoundOnce && abandon ? load_abandon_bound(ctrl, mode) : kInf;
+    auto bound = [&]() {
+        return kBoundOnce ? thr_once
+                          : (mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl));
+    };
     float lo_c = 0.0f, up_c = 0.0f, lo_n = 0.0f, up_n = 0.0f, P = 0.0f;
     if constexpr (LB) {
         // The bound may have fallen since the cascade kept this window.
-        alive = alive && !(lbt > prune_limit(thr, m, L2));
+        alive = alive && !(lbt > prune_limit(bound(), m, L2));
         if (!__any(alive)) return kInf;
         lo_c = elo[lane < m ? lane : m - 1];
         up_c = eup[lane < m ? lane : m - 1];
@@ -245,6 +255,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
             float cm = E[0];
 #pragma unroll
             for (int t = 1; t < B; ++t) cm = fminf(cm, E[t]);
+            const float thr = bound();
             if constexpr (LB) {
                 const float rest = lbt - P;
                 const float bound = cm + (rest > 0.0f ? rest : 0.0f);
@@ -359,16 +370,29 @@ inline int resident_waves(const void* fn) {
     return regs > 0 ? 512 / regs : 0;
 }
 
+// Bytes of scratch memory (register spills) a kernel needs per lane, or -1.
+inline int spill_bytes(const void* fn) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, fn) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    return static_cast<int>(attr.localSizeBytes);
+}
+
 // The cumulative-bound variant holds a few more registers; it is used unless
 // that drops residency below min(2, the plain variant's) waves per SIMD, where
-// the lost latency hiding costs more than the earlier abandon saves.
+// the lost latency hiding costs more than the earlier abandon saves, or makes
+// it spill registers to scratch memory where the plain variant does not.
 template <int W, bool L2>
 bool band_lb_pays() {
     static const bool pays = [] {
-        const int with = resident_waves(reinterpret_cast<const void*>(&band_kernel<W, L2, true>));
-        const int without =
-            resident_waves(reinterpret_cast<const void*>(&band_kernel<W, L2, false>));
-        return with > 0 && with >= (without < 2 ? without : 2);
+        const void* fw = reinterpret_cast<const void*>(&band_kernel<W, L2, true>);
+        const void* fo = reinterpret_cast<const void*>(&band_kernel<W, L2, false>);
+        const int with = resident_waves(fw);
+        const int without = resident_waves(fo);
+        const int sw = spill_bytes(fw), so = spill_bytes(fo);
+        return with > 0 && with >= (without < 2 ? without : 2) && sw >= 0 && sw <= so;
     }();
     return pays;
 }
