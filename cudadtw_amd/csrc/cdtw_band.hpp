@@ -44,7 +44,7 @@ constexpr int kBandSlideMax = 54;
 // W = 111..112: one, which keeps them within 256 registers, i.e. two waves
 // per SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which
 // keeps the 2W + 1 band values in vector + accumulation registers (no scratch
-// memory): two waves per SIMD up to W = 113, one from W = 114.
+// memory): two waves per SIMD up to W = 114, one from W = 115.
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 
