@@ -181,11 +181,15 @@ __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
 // + 128, so the steps 64c .. 64c + 63 (which read chunks c and c + 1) read
 // slots (64c mod 128) + l + u for u < 64 with no wrap: one ds_read with an
 // immediate offset per step.  After those steps chunk c + 2 replaces chunk
-// c.  The query envelope of those 64 steps is loaded lane-distributed
-// (lane l holds lo/up[64c + l]) and broadcast per step with readlane, so the
-// inner loop issues no global or scalar loads.  A wave reads only its own
-// ring and LDS operations of one wave complete in order; a wave-level fence
-// orders each refill before the reads that follow.
+// c.  The query envelope is wave-uniform, so each block of 16 steps reads it
+// with scalar loads (two 16-value loads) and uses it as a scalar operand:
+// 6 VALU per element (sub, mul, a move for med3's second scalar, med3, sub,
+// fma).  Broadcasting a lane-distributed copy with readlane instead cost two
+// more VALU per element and 13% of the C2 search.  The steps past the last
+// whole block of 16 still use the lane-distributed copy (lane l holds
+// lo/up[64c + l]) and readlane.  A wave reads only its own ring and LDS
+// operations of one wave complete in order; a wave-level fence orders each
+// refill before the reads that follow.
 constexpr int kCascadeWaves = 4;
 constexpr int kRing = 256;
 // Survivors are staged per wave in LDS and appended kCandBuf - 64 or more at
@@ -286,10 +290,12 @@ __global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
             const int32_t nstep = m - i0 < kWave ? m - i0 : kWave;
             int32_t u = 0;
             for (; u + 16 <= nstep; u += 16) {
+                // envelope of these 16 steps: wave-uniform scalar loads
+                const float* __restrict__ lq = lo + (i0 + u);
+                const float* __restrict__ uq = up + (i0 + u);
 #pragma unroll
                 for (int t = 0; t < 16; ++t)
-                    acc = keogh_step<L2>(rp[u + t], mu, rs, bcast(lv, u + t), bcast(uv, u + t),
-                                         acc);
+                    acc = keogh_step<L2>(rp[u + t], mu, rs, lq[t], uq[t], acc);
                 if (!__any(alive && !(acc > lim))) {
                     gone = true;
                     break;
