@@ -97,9 +97,13 @@ def test_search_with_given_stats_and_wide_band(engine, w):
     assert (loc, dist) == (r.location, r.distance)
 
 
+# (1024, 16) / (300, 3) / (1024, 0): k up to MDTW_MAX_K, so the k-th smallest
+# seed (a radix selection over up to 16384 seed distances) sets the first
+# bound; excl 0 and 3 give more than 2^15 buckets, the multi-block tighten.
 @pytest.mark.parametrize("algo,k,excl", [(ALGO_CDTW, 16, 64), (ALGO_CDTW, 4, 0),
                                          (ALGO_LDTW, 8, 64), (ALGO_ED, 8, 64), (ALGO_CDTW, 1, 0),
-                                         (ALGO_LDTW, 1, 0)])
+                                         (ALGO_LDTW, 1, 0), (ALGO_CDTW, 1024, 16),
+                                         (ALGO_CDTW, 300, 3), (ALGO_ED, 1024, 0)])
 def test_topk_equals_greedy_over_profile(engine, algo, k, excl):
     m, w = 128, 13
     s = synth.ecg_like(1 << 17, 31)
@@ -118,6 +122,25 @@ def test_topk_equals_greedy_over_profile(engine, algo, k, excl):
     assert np.array_equal(r.positions[:len(idx)], idx)
     assert np.array_equal(r.distances[:len(idx)], np.sqrt(d2).astype(np.float32))
     assert r.info["found"] == len(idx)
+
+
+def test_topk_more_picks_than_seeds(engine):
+    # A short stream where fewer seed windows (>= 2*excl apart) exist than k:
+    # no seed bound (every window is a candidate), and the greedy selection
+    # may find fewer than k matches.
+    m, w, k, excl = 64, 6, 64, 32
+    s = synth.ecg_like(4000, 33)
+    zq = O.znormalize_f32(synth.ecg_query(m, 2))
+    engine.set_query(zq, w)
+    ds = dev(s)
+    avg, sd = engine.avg_std(ds, m)
+    prof = host(engine.gpu_cdtw(ds, avg, sd, w))
+    r = engine.search(ds, w, k=k, excl=excl)
+    idx, d2 = O.topk_greedy(prof, k, excl)
+    assert r.info["seeded"] < k
+    assert r.info["found"] == len(idx)
+    assert np.array_equal(r.positions[:len(idx)], idx)
+    assert np.array_equal(r.distances[:len(idx)], np.sqrt(d2).astype(np.float32))
 
 
 def test_metal_topk16_ldtw(engine):
