@@ -44,9 +44,55 @@ constexpr int kBandSlideMax = 54;
 // W = 111..112: one, which keeps them within 256 registers, i.e. two waves
 // per SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which
 // keeps the 2W + 1 band values in vector + accumulation registers (no scratch
-// memory): two waves per SIMD up to W = 114, one from W = 115.
+// memory): two waves per SIMD up to W = 114, one from W = 137; W = 115..136
+// keep part of the band in LDS instead (below) at two waves per SIMD.
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
+// Bands W = 115..kBandTailMaxW keep band values 0..kBandTailR-1 in vector
+// registers and the rest (the tail, t >= kBandTailR) in LDS, 16 bytes per
+// lane per chunk of four values: at most 16 chunks, 64 KiB per workgroup, so
+// two workgroups (two waves per SIMD, the register budget of 256) share a
+// CU's 160 KiB.  Without the tail these bands need accumulation registers
+// and run at one wave per SIMD, where each column's serial min3 -> fmac
+// chain is exposed (half the rate).  Their kernels are compiled for two
+// waves per SIMD (band_kernel_tail); the compiler then spills a few band
+// values per column to scratch memory, a number that grows with the tail:
+// measured at M = 1024, the tail kernel runs W = 115 at 18 T cells/s (one
+// wave per SIMD without it: 10), 120 at 16 (9), 128 at 12 (9), 136 at 10
+// (9) and 143 at 8 (9), hence the limit.
+constexpr int kBandTailR = 212;
+constexpr int kBandTailMaxW = 136;
+__host__ __device__ constexpr int band_tail_len(int w) {
+    return (w > 114 && w <= kBandTailMaxW) ? 2 * w + 1 - kBandTailR : 0;
+}
+constexpr int kBandTailStride = 4 * 64;  // float4 slots between a lane's chunks
+
+template <int NCHUNK>
+__device__ __forceinline__ float4* band_tail_lds() {
+    __shared__ float4 buf[NCHUNK * kBandTailStride];
+    return buf + threadIdx.x;
+}
+// Store of a tail chunk, pinned where it is written (a plain store was
+// deferred by the compiler, which then held the chunks in registers and
+// spilled them).  LDS operations of a wave complete in order, so later reads
+// of the chunk see it; the compiler's own waits only become conservative.
+typedef float band_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void band_tail_store(float4* p, const float4& v) {
+    const band_v4f w = {v.x, v.y, v.z, v.w};
+    asm volatile("ds_write_b128 %0, %1"
+                 :
+                 : "v"(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p))), "v"(w)
+                 : "memory");
+}
+__device__ __forceinline__ float f4_get(const float4& v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ void f4_set(float4& v, int i, float f) {
+    if (i == 0) v.x = f;
+    else if (i == 1) v.y = f;
+    else if (i == 2) v.z = f;
+    else v.w = f;
+}
 
 // Next 64 windows from the shared cursor (one-wave workgroups).
 __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane) {
@@ -125,10 +171,25 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     // of order, so every wait is for all of them.)  Wider bands read the
     // query in chunks of CH scalars per column.
     constexpr bool kSlide = B + 2 * U - 1 <= kBandSlideMax;
-    float E[B];
+    // KT band values in the LDS tail (0 for most bands), BR in registers.
+    constexpr int KT = band_tail_len(W);
+    constexpr int KT4 = (KT + 3) / 4;
+    constexpr int BR = B - KT;
+    static_assert(KT == 0 || (U == 1 && !kSlide && W < BR), "tail layout");
+    float E[BR];
 #pragma unroll
-    for (int t = 0; t < B; ++t) E[t] = kInf;
+    for (int t = 0; t < BR; ++t) E[t] = kInf;
     E[W] = 0.0f;  // D[-1][-1]
+    [[maybe_unused]] float4* tl = nullptr;
+    // minimum of the tail's values after the last column (for the abandon
+    // check, which then needs no LDS reads)
+    [[maybe_unused]] float tail_min = kInf;
+    const float4 inf4 = make_float4(kInf, kInf, kInf, kInf);
+    if constexpr (KT4 > 0) {
+        tl = band_tail_lds<KT4>();
+#pragma unroll
+        for (int c = 0; c < KT4; ++c) tl[c * kBandTailStride] = inf4;
+    }
 
     const float y = 1.0f / sg;
     const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
@@ -213,6 +274,63 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
             for (int k = 0; k + U < R; ++k) qs[k] = qs[k + U];
 #pragma unroll
             for (int u = 0; u < U; ++u) qs[R - U + u] = pf[u];
+        } else if constexpr (KT4 > 0) {
+            // One column (U == 1): cells 0..BR-1 in registers, then the tail
+            // in chunks of four, each read from LDS two chunks ahead (its
+            // first value is the last register cell's right neighbour, or the
+            // previous chunk's), updated in registers and written back.
+            const float* __restrict__ qc = qx + j0;
+            const float zj = z[0];
+            float prev = kInf;
+            float4 T0 = tl[0];
+            float4 T1 = KT4 > 1 ? tl[kBandTailStride] : inf4;
+            float4 T2 = KT4 > 2 ? tl[2 * kBandTailStride] : inf4;
+            __builtin_amdgcn_sched_barrier(0x16);
+            tail_min = kInf;
+            float qa[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) qa[i] = qc[i];
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) {
+                float qb[CH];
+                if (ch + 1 < NC) {
+#pragma unroll
+                    for (int i = 0; i < CH; ++i) qb[i] = qc[(ch + 1) * CH + i];
+                }
+                __builtin_amdgcn_sched_barrier(0x96);
+#pragma unroll
+                for (int i = 0; i < CH; ++i) {
+                    const int t = ch * CH + i;
+                    if (t < BR) {
+                        const float nxt = t + 1 < BR ? E[t + 1] : T0.x;
+                        const float mn = fminf(fminf(prev, E[t]), nxt);
+                        const float d = qa[i] - zj;
+                        const float e = L2 ? fmaf(d, d, mn) : fabsf(d) + mn;
+                        E[t] = e;
+                        prev = e;
+                    } else if (t < B) {
+                        const int k = t - BR, e4 = k & 3, c4 = k >> 2;
+                        const float nxt = e4 < 3 ? f4_get(T0, e4 + 1) : T1.x;
+                        const float mn = fminf(fminf(prev, f4_get(T0, e4)), nxt);
+                        const float d = qa[i] - zj;
+                        const float e = L2 ? fmaf(d, d, mn) : fabsf(d) + mn;
+                        f4_set(T0, e4, e);
+                        prev = e;
+                        if (e4 == 3 || t == B - 1) {
+                            tail_min = fminf(tail_min, fminf(fminf(T0.x, T0.y), fminf(T0.z, T0.w)));
+                            band_tail_store(tl + c4 * kBandTailStride, T0);
+                            T0 = T1;
+                            T1 = T2;
+                            T2 = c4 + 3 < KT4 ? tl[(c4 + 3) * kBandTailStride] : inf4;
+                            // LDS traffic stays where it is written: hoisted
+                            // reads would hold more of the tail in registers
+                            __builtin_amdgcn_sched_barrier(0x16);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < CH; ++i) qa[i] = qb[i];
+            }
         } else {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -254,7 +372,10 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
         if (abandon && ((j0 / U) % (kBandCheck / U)) == (kBandCheck / U) - 1) {
             float cm = E[0];
 #pragma unroll
-            for (int t = 1; t < B; ++t) cm = fminf(cm, E[t]);
+            for (int t = 1; t < BR; ++t) cm = fminf(cm, E[t]);
+            if constexpr (KT4 > 0) {
+                cm = fminf(cm, tail_min);
+            }
             const float thr = bound();
             if constexpr (LB) {
                 const float rest = lbt - P;
@@ -275,7 +396,7 @@ __device__ __forceinline__ uint32_t band_lane_rank(unsigned long long mask) {
 }
 
 template <int W, bool L2, bool LB>
-__global__ __launch_bounds__(256) void band_kernel(
+__device__ __forceinline__ void band_kernel_body(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
     const float* __restrict__ qx, const int32_t* __restrict__ idx, float* __restrict__ out,
     unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, BandScalars a,
@@ -358,6 +479,33 @@ __global__ __launch_bounds__(256) void band_kernel(
     }
 }
 
+#define MDTW_BAND_KERNEL_ARGS                                                                 \
+    const float *__restrict__ s, const float *__restrict__ avg, const float *__restrict__ sd,   \
+        const float *__restrict__ qx, const int32_t *__restrict__ idx, float *__restrict__ out, \
+        unsigned long long *__restrict__ res, Ctrl *__restrict__ ctrl, BandScalars a,            \
+        const float *__restrict__ lbt, const float *__restrict__ elo, const float *__restrict__ eup
+
+template <int W, bool L2, bool LB>
+__global__ __launch_bounds__(256) void band_kernel(MDTW_BAND_KERNEL_ARGS) {
+    band_kernel_body<W, L2, LB>(s, avg, sd, qx, idx, out, res, ctrl, a, lbt, elo, eup);
+}
+
+// Bands with an LDS tail, compiled for two waves per SIMD (see kBandTailMaxW).
+template <int W, bool L2, bool LB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void band_kernel_tail(
+    MDTW_BAND_KERNEL_ARGS) {
+    band_kernel_body<W, L2, LB>(s, avg, sd, qx, idx, out, res, ctrl, a, lbt, elo, eup);
+}
+#undef MDTW_BAND_KERNEL_ARGS
+
+template <int W, bool L2, bool LB>
+constexpr auto band_kernel_fn() {
+    if constexpr (band_tail_len(W) > 0)
+        return &band_kernel_tail<W, L2, LB>;
+    else
+        return &band_kernel<W, L2, LB>;
+}
+
 // Waves per SIMD a kernel can keep resident, from its register count
 // (512 registers per lane per SIMD, allocated in granules of 8).
 inline int resident_waves(const void* fn) {
@@ -387,8 +535,8 @@ inline int spill_bytes(const void* fn) {
 template <int W, bool L2>
 bool band_lb_pays() {
     static const bool pays = [] {
-        const void* fw = reinterpret_cast<const void*>(&band_kernel<W, L2, true>);
-        const void* fo = reinterpret_cast<const void*>(&band_kernel<W, L2, false>);
+        const void* fw = reinterpret_cast<const void*>(band_kernel_fn<W, L2, true>());
+        const void* fo = reinterpret_cast<const void*>(band_kernel_fn<W, L2, false>());
         const int with = resident_waves(fw);
         const int without = resident_waves(fo);
         const int sw = spill_bytes(fw), so = spill_bytes(fo);
@@ -404,12 +552,12 @@ int band_launch(hipStream_t st, unsigned grid, const BandArgs& a) {
     const bool lb = a.lbt && a.idx && a.lo && a.up &&
                     (a.mode == kModeBest || a.mode == kModeCollect) && band_lb_pays<W, L2>();
     if (lb)
-        hipLaunchKernelGGL((band_kernel<W, L2, true>), dim3(grid), dim3(kBandWaves * kWave), 0,
-                           st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, a.lbt,
+        hipLaunchKernelGGL((band_kernel_fn<W, L2, true>()), dim3(grid), dim3(kBandWaves * kWave),
+                           0, st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, a.lbt,
                            a.lo, a.up);
     else
-        hipLaunchKernelGGL((band_kernel<W, L2, false>), dim3(grid), dim3(kBandWaves * kWave), 0,
-                           st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, nullptr,
+        hipLaunchKernelGGL((band_kernel_fn<W, L2, false>()), dim3(grid), dim3(kBandWaves * kWave),
+                           0, st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, nullptr,
                            nullptr, nullptr);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {

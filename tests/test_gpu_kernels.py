@@ -87,7 +87,8 @@ def test_stats_refcompat_matches_restatement(engine):
 @pytest.mark.parametrize("m,w", [(128, 13), (128, 0), (128, 1), (128, 26), (256, 51), (100, 29),
                                  (100, 30), (256, 59), (256, 60), (256, 89), (256, 90),
                                  (256, 112), (37, 5), (11, 10), (5, 2), (3, 1), (2, 1),
-                                 (300, 113), (300, 144), (400, 145), (400, 177), (500, 208),
+                                 (300, 113), (300, 114), (300, 115), (300, 120), (400, 136),
+                                 (400, 137), (300, 144), (400, 145), (400, 177), (500, 208),
                                  (500, 209), (600, 240), (700, 241)])
 def test_cdtw_bit_exact_random_order(engine, m, w):
     n = 6000

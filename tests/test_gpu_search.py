@@ -81,7 +81,7 @@ def test_refcompat_search_matches_restated_reference(engine):
     assert abs(r.distance / np.sqrt(prof.min()) - 1) < 1e-5
 
 
-@pytest.mark.parametrize("w", [13, 51, 150])
+@pytest.mark.parametrize("w", [13, 51, 120, 136, 150])
 def test_search_with_given_stats_and_wide_band(engine, w):
     s = synth.ecg_like(1 << 17, 21)
     q, pos = synth.cut_query(s, 512, 22)
@@ -95,6 +95,11 @@ def test_search_with_given_stats_and_wide_band(engine, w):
     assert r.distances[0] == np.float32(np.sqrt(prof.min()))
     loc, dist = engine.prune_cdtw(ds, avg, sd, w)
     assert (loc, dist) == (r.location, r.distance)
+    # top-k through the same DP (abandoning against the top-k bound)
+    rk = engine.search(ds, w, k=4, excl=256, avg=avg, sd=sd)
+    idx, d2 = O.topk_greedy(prof, 4, 256)
+    assert np.array_equal(rk.positions[:len(idx)], idx)
+    assert np.array_equal(rk.distances[:len(idx)], np.sqrt(d2).astype(np.float32))
 
 
 # (1024, 16) / (300, 3) / (1024, 0): k up to MDTW_MAX_K, so the k-th smallest
