@@ -72,18 +72,6 @@ __device__ __forceinline__ float4* band_tail_lds() {
     __shared__ float4 buf[NCHUNK * kBandTailStride];
     return buf + threadIdx.x;
 }
-// Store of a tail chunk, pinned where it is written (a plain store was
-// deferred by the compiler, which then held the chunks in registers and
-// spilled them).  LDS operations of a wave complete in order, so later reads
-// of the chunk see it; the compiler's own waits only become conservative.
-typedef float band_v4f __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void band_tail_store(float4* p, const float4& v) {
-    const band_v4f w = {v.x, v.y, v.z, v.w};
-    asm volatile("ds_write_b128 %0, %1"
-                 :
-                 : "v"(static_cast<uint32_t>(reinterpret_cast<uintptr_t>(p))), "v"(w)
-                 : "memory");
-}
 __device__ __forceinline__ float f4_get(const float4& v, int i) {
     return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
 }
@@ -318,7 +306,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                         prev = e;
                         if (e4 == 3 || t == B - 1) {
                             tail_min = fminf(tail_min, fminf(fminf(T0.x, T0.y), fminf(T0.z, T0.w)));
-                            band_tail_store(tl + c4 * kBandTailStride, T0);
+                            tl[c4 * kBandTailStride] = T0;
                             T0 = T1;
                             T1 = T2;
                             T2 = c4 + 3 < KT4 ? tl[(c4 + 3) * kBandTailStride] : inf4;
