@@ -230,7 +230,7 @@ int capture(mdtw_batch* b, const float* s, int64_t n, int32_t metric, int32_t k,
     for (int32_t i = 0; i < b->nq; ++i) {
         const int32_t w = b->w[i];
         const double rate = w <= kBandMaxW ? 1.0
-                            : (metric == MDTW_METRIC_L2 && w <= kBandWideMaxW) ? 2.2
+                            : w <= kBandWideMaxW ? 2.2
                                                                              : 6.5;
         cost[i] = static_cast<double>(n - b->m[i] + 1) * band_cells(b->m[i], w) * rate;
     }

@@ -1,9 +1,8 @@
 // cdtw.hip -- banded cDTW dispatch and the wide-band kernel.
 //
-// W <= kBandMaxW (both metrics) and W <= kBandWideMaxW (L2): the
-// register-band kernel (cdtw_band.hpp), one lane per window.  Wider bands (up
-// to kMaxBandW), and L1 bands beyond kBandMaxW, use the anti-diagonal kernel
-// below.
+// W <= kBandWideMaxW (both metrics): the register-band kernel
+// (cdtw_band.hpp), one lane per window.  Wider bands (up to kMaxBandW) use
+// the anti-diagonal kernel below.
 //
 // Anti-diagonal kernel.  One wave per window.  Diagonal k = i + j holds the
 // cells with offset d = i - j in [-W, W], d = k (mod 2); slot p of diagonal k
@@ -240,11 +239,11 @@ int wide_dispatch(hipStream_t st, unsigned grid, const BandArgs& a, int32_t w, b
 BandLaunchFn band_lookup(int w, bool l2) {
     if (w < 0) return nullptr;
     if (w > kBandMaxW) {
-        if (!l2 || w > kBandWideMaxW) return nullptr;
-        if (w <= 144) return band_table_4(w);
-        if (w <= 176) return band_table_5(w);
-        if (w <= 208) return band_table_6(w);
-        return band_table_7(w);
+        if (w > kBandWideMaxW) return nullptr;
+        if (w <= 144) return l2 ? band_table_4(w) : band_table_8(w);
+        if (w <= 176) return l2 ? band_table_5(w) : band_table_9(w);
+        if (w <= 208) return l2 ? band_table_6(w) : band_table_10(w);
+        return l2 ? band_table_7(w) : band_table_11(w);
     }
     if (w < 30) return band_table_0(w, l2);
     if (w < 60) return band_table_1(w, l2);

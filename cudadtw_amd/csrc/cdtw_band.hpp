@@ -42,7 +42,7 @@ constexpr int kBandWaves = 4;  // waves per workgroup
 constexpr int kBandSlideMax = 54;
 // Bands up to kBandMaxW unroll kBandU columns per block (W = 108..110: two,
 // W = 111..112: one, which keeps them within 256 registers, i.e. two waves
-// per SIMD); wider bands (L2, up to kBandWideMaxW) unroll one column, which
+// per SIMD); wider bands (up to kBandWideMaxW) unroll one column, which
 // keeps the 2W + 1 band values in vector + accumulation registers (no scratch
 // memory): two waves per SIMD up to W = 114, one from W = 137; W = 115..136
 // keep part of the band in LDS instead (below) at two waves per SIMD.
@@ -564,5 +564,9 @@ BandLaunchFn band_table_4(int w);           // L2, W 113..144
 BandLaunchFn band_table_5(int w);           // L2, W 145..176
 BandLaunchFn band_table_6(int w);           // L2, W 177..208
 BandLaunchFn band_table_7(int w);           // L2, W 209..kBandWideMaxW
+BandLaunchFn band_table_8(int w);           // L1, W 113..144
+BandLaunchFn band_table_9(int w);           // L1, W 145..176
+BandLaunchFn band_table_10(int w);          // L1, W 177..208
+BandLaunchFn band_table_11(int w);          // L1, W 209..kBandWideMaxW
 
 }  // namespace mdtw

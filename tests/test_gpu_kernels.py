@@ -116,6 +116,23 @@ def test_cdtw_wide_kernel_bit_exact(engine, w):
         assert np.array_equal(got2, ref)
 
 
+@pytest.mark.parametrize("m,w", [(300, 112), (300, 113), (300, 115), (400, 136), (400, 137),
+                                 (500, 200), (600, 240), (700, 241)])
+def test_cdtw_l1_wide_bands_bit_exact(engine, m, w):
+    # L1 bands above 112: register band (LDS tail at 115..136) to W = 240,
+    # the anti-diagonal kernel beyond; both against the restated DP.
+    n = 4000
+    s = synth.random_walk(n, m + w + 7)
+    zq, ds, avg, sd, ha, hs = prepare(engine, s, synth.random_walk(m, 98), w)
+    L = n - m + 1
+    idx = np.random.default_rng(1).permutation(L).astype(np.int32)
+    got = host(engine.gpu_cdtw(ds, avg, sd, w, indices=dev(idx, torch.int32), metric=METRIC_L1))
+    ref = O.cdtw(zq, s, ha, hs, w, idx=idx, metric=0, fused=1)
+    assert np.array_equal(got, ref), np.max(np.abs(got - ref))
+    wide = host(engine.gpu_cdtw(ds, avg, sd, w, metric=METRIC_L1, wide=True))
+    assert np.array_equal(wide[idx], ref)
+
+
 @pytest.mark.parametrize("wide", [False, True])
 def test_cdtw_l1_bit_exact(engine, wide):
     m, w, n = 64, 6, 4000
