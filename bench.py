@@ -156,6 +156,32 @@ def section_c2(eng, torch, args):
     re_, t_ed = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=5)
     log(f"[c2] ED search: {t_ed*1e3:.2f} ms, location {re_.location}")
 
+    # The memory-side kernels over every window of the 2^27 stream, timed with
+    # HIP events on the stream they run on: window statistics (reads 4 B,
+    # writes 8 B per window), LB_Kim and LB_Keogh and the ED profile (read the
+    # sample plus mean and sigma, 12 B, write 4 B); GB/s of these algorithmic
+    # bytes against the 8 TB/s HBM peak.
+    eng.set_query(zq, w)
+    mem = {}
+    for name, nbytes, fn in (
+            ("stats", 12, lambda: eng.avg_std(ds, m, out=(avg, sd))),
+            ("lb_kim", 16, lambda: eng.lb_kim(ds, avg, sd)),
+            ("lb_keogh", 16, lambda: eng.lb_keogh(ds, avg, sd)),
+            ("ed_profile", 16, lambda: eng.ed(ds, avg, sd))):
+        fn()
+        st.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(3):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        gbps = L * nbytes / (ms / 1e3) / 1e9
+        mem[name] = {"ms": round(ms, 3), "gb_per_s": round(gbps, 1),
+                     "frac_of_8tbs": round(gbps / 8000.0, 3), "bytes_per_window": nbytes}
+    log(f"[c2] memory-side kernels over all windows: {mem}")
+
     # DP rate on wide bands (|Q| = 1024 at 10 / 15.6 / 20 %: W = 102 in the
     # four-column register kernel, 160 and 205 in the one-wave-per-SIMD wide
     # register kernel), profile mode over a 2^24-sample prefix of the stream.
@@ -222,6 +248,7 @@ def section_c2(eng, torch, args):
         "gcups_by_band": {str(k): round(v["gcups"], 2) for k, v in results.items()},
         "dp_kernel_gcups_by_band": {str(k): round(v["dp_gcups"], 2) for k, v in results.items()},
         "dp_kernel_gcups_wide_m1024": wide,
+        "memory_kernels_all_windows": mem,
         "pruned_search": {"ms": round(t_search * 1e3, 3), "windows_per_s": round(L / t_search, 1),
                           "effective_gcups": round(L * cells(m, w) / t_search / 1e9, 2),
                           "location": r.location, "distance": round(r.distance, 6),

@@ -156,9 +156,69 @@ __global__ __launch_bounds__(256) void lb_keogh_kernel(
     const int64_t w = idx ? idx[g] : g;
     const float* x = s + w;
     const float mu = avg[w], sg = sd[w];
+    // z = (x - mu) / sigma as the exact IEEE quotient without a division
+    // per element (band_z); the envelope index is wave-uniform, so blocks of
+    // 16 read it with scalar loads.
+    const float y = 1.0f / sg;
+    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
     float acc = 0.0f;
-    for (int32_t i = 0; i < m; ++i) {
-        const float z = (x[i] - mu) / sg;
+    int32_t i = 0;
+    for (; i + 16 <= m; i += 16) {
+        const float* __restrict__ lq = lo + i;
+        const float* __restrict__ uq = up + i;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const float z = band_z(x[i + t], mu, sg, y, fast);
+            const float e = z - __builtin_amdgcn_fmed3f(z, lq[t], uq[t]);
+            acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+        }
+    }
+    for (; i < m; ++i) {
+        const float z = band_z(x[i], mu, sg, y, fast);
+        const float e = z - __builtin_amdgcn_fmed3f(z, lo[i], up[i]);
+        acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+    }
+    out[g] = acc;
+}
+
+// All windows: a workgroup's 256 consecutive windows read samples
+// [g0, g0 + 255 + M) of the stream, staged once in LDS with coalesced loads
+// (dynamic shared memory, (255 + M) floats); each lane then reads its
+// window from LDS instead of issuing one global load per element.
+template <bool L2>
+__global__ __launch_bounds__(256) void lb_keogh_all_kernel(
+    const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
+    const float* __restrict__ lo, const float* __restrict__ up, int32_t m, int64_t count,
+    float* __restrict__ out) {
+    extern __shared__ float seg[];
+    const int64_t g0 = static_cast<int64_t>(blockIdx.x) * 256;
+    const int64_t last = count + m - 2;  // last stream sample any window reads
+    const int32_t len = 255 + m;
+    for (int32_t k = threadIdx.x; k < len; k += 256) {
+        const int64_t gi = g0 + k;
+        seg[k] = s[gi < last ? gi : last];
+    }
+    __syncthreads();
+    const int64_t g = g0 + threadIdx.x;
+    if (g >= count) return;
+    const float* x = seg + threadIdx.x;
+    const float mu = avg[g], sg = sd[g];
+    const float y = 1.0f / sg;
+    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
+    float acc = 0.0f;
+    int32_t i = 0;
+    for (; i + 16 <= m; i += 16) {
+        const float* __restrict__ lq = lo + i;
+        const float* __restrict__ uq = up + i;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const float z = band_z(x[i + t], mu, sg, y, fast);
+            const float e = z - __builtin_amdgcn_fmed3f(z, lq[t], uq[t]);
+            acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+        }
+    }
+    for (; i < m; ++i) {
+        const float z = band_z(x[i], mu, sg, y, fast);
         const float e = z - __builtin_amdgcn_fmed3f(z, lo[i], up[i]);
         acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
     }
@@ -380,6 +440,16 @@ int launch_lb_keogh(hipStream_t st, const float* s, const float* avg, const floa
                     int64_t count, int32_t metric, float* out) {
     if (count <= 0) return MDTW_OK;
     const unsigned nb = static_cast<unsigned>((count + 255) / 256);
+    if (!idx) {
+        const size_t lds = sizeof(float) * static_cast<size_t>(255 + m);
+        if (metric)
+            hipLaunchKernelGGL(lb_keogh_all_kernel<true>, dim3(nb), dim3(256), lds, st, s, avg,
+                               sd, lo, up, m, count, out);
+        else
+            hipLaunchKernelGGL(lb_keogh_all_kernel<false>, dim3(nb), dim3(256), lds, st, s, avg,
+                               sd, lo, up, m, count, out);
+        return check_launch();
+    }
     if (metric)
         hipLaunchKernelGGL(lb_keogh_kernel<true>, dim3(nb), dim3(256), 0, st, s, avg, sd, lo,
                            up, m, idx, count, out);

@@ -110,18 +110,6 @@ __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t first
     return static_cast<int64_t>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
-// z = (x - mu) / sigma, bit-identical to the IEEE quotient: q = a * y with y
-// the correctly rounded 1/sigma, then one Markstein correction
-// fma(fma(-sigma, q, a), y, q).  Outside the exponent range where the
-// residual a - sigma*q is exact (tiny |a|, extreme sigma) the lane divides.
-__device__ __forceinline__ float band_z(float x, float mu, float sg, float y, bool fast) {
-    const float a = x - mu;
-    const float q = a * y;
-    float z = fmaf(fmaf(-sg, q, a), y, q);
-    if (__builtin_expect(!fast || (a != 0.0f && fabsf(a) < 0x1p-100f), 0)) z = a / sg;
-    return z;
-}
-
 // One column of the band update with the query values of this column in
 // q[0..B) (wave-uniform scalars).
 template <int B, bool L2, class Q>

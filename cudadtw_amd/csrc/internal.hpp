@@ -396,6 +396,18 @@ __device__ __forceinline__ float load_abandon_bound(const Ctrl* c, int mode) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// z = (x - mu) / sigma, bit-identical to the IEEE quotient: q = a * y with y
+// the correctly rounded 1/sigma, then one Markstein correction
+// fma(fma(-sigma, q, a), y, q).  Outside the exponent range where the
+// residual a - sigma*q is exact (tiny |a|, extreme sigma) the lane divides.
+__device__ __forceinline__ float band_z(float x, float mu, float sg, float y, bool fast) {
+    const float a = x - mu;
+    const float q = a * y;
+    float z = fmaf(fmaf(-sg, q, a), y, q);
+    if (__builtin_expect(!fast || (a != 0.0f && fabsf(a) < 0x1p-100f), 0)) z = a / sg;
+    return z;
+}
+
 // Exact distance d of window w is known: lower its bucket's minimum.
 __device__ __forceinline__ void note_bucket(uint32_t* bmin, long long bucket_len, int64_t w,
                                             float d) {
