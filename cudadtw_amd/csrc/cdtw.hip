@@ -53,101 +53,132 @@ __device__ __forceinline__ float wave_min_f32(float v) {
     return v;
 }
 
-template <int C, bool L2>
-__device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
-                         const float* __restrict__ q, int32_t m, int32_t W, int mode,
-                         const Ctrl* ctrl, bool& alive) {
+// One diagonal of parity PAR: N = the new diagonal written over A2 (the
+// diagonal two back, whose slot c is the only one of it slot c reads), A1
+// the previous diagonal.
+template <int C, bool L2, int PAR>
+__device__ __forceinline__ void wide_step(float (&A2)[C], const float (&A1)[C],
+                                          const float (&Qv)[C], const float (&Zv)[C],
+                                          const float (&pad)[C]) {
+    if constexpr (PAR == 0) {
+        const float in = lane_from_below(A1[C - 1], kInf);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const float up = c > 0 ? A1[c - 1] : in;
+            const float mn = fminf(fminf(A2[c], up), A1[c]);
+            const float dd = Qv[c] - Zv[c];
+            const float v = L2 ? fmaf(dd, dd, mn) : fabsf(dd) + mn;
+            A2[c] = fmaxf(v, pad[c]);
+        }
+    } else {
+        const float in = lane_from_above(A1[0], kInf);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const float lf = c + 1 < C ? A1[c + 1] : in;
+            const float mn = fminf(fminf(A2[c], A1[c]), lf);
+            const float dd = Qv[c] - Zv[c];
+            const float v = L2 ? fmaf(dd, dd, mn) : fabsf(dd) + mn;
+            A2[c] = fmaxf(v, pad[c]);
+        }
+    }
+}
+
+// Diagonals are processed in pairs, the first of parity FIRST = W & 1 (the
+// parity of diagonal 0), so each position's parity is fixed at compile time
+// and the two newest diagonals alternate between the arrays X and Y instead
+// of being copied.  Query values enter on parity-0 steps and stream values on
+// parity-1 steps; their indices grow by one per pair (query: never below 0,
+// since 64C > W), and values past the end come from the padded 64-value
+// buffers (+inf / -inf), so a pair costs only a few scalar instructions: the
+// one-diagonal form spent ~41 scalar instructions per diagonal (parity,
+// index arithmetic and bounds branches), and the scalar unit, shared by a
+// CU's four SIMDs, bounded the kernel.
+template <int C, bool L2, int FIRST>
+__device__ float wide_dp_pairs(const float* __restrict__ x, float mu, float sg,
+                               const float* __restrict__ q, int32_t m, int32_t W, int mode,
+                               const Ctrl* ctrl, bool& alive) {
     const int lane = threadIdx.x;
     const int P = W + 1;
-    float pad0[C], pad1[C], A1[C], A2[C], Qv[C], Zv[C];
+    float pad0[C], pad1[C], X[C], Y[C], Qv[C], Zv[C];
     const int pfin = W >> 1;
-    const int par0 = W & 1;
 #pragma unroll
     for (int c = 0; c < C; ++c) {
         const int p = lane * C + c;
         pad0[c] = p < P ? 0.0f : kInf;
         pad1[c] = p < P - 1 ? 0.0f : kInf;
-        A1[c] = kInf;
-        A2[c] = p == pfin ? 0.0f : kInf;  // D[-1][-1] on diagonal -2
-        const int d = -W + 2 * p + par0;  // diagonal 0: i = d/2, j = -d/2
+        Y[c] = kInf;                       // diagonal -1
+        X[c] = p == pfin ? 0.0f : kInf;    // diagonal -2: D[-1][-1]
+        const int d = -W + 2 * p + FIRST;  // diagonal 0: i = d/2, j = -d/2
         const int i = d / 2, j = -i;
         Qv[c] = (i >= 0 && i < m) ? q[i] : kInf;
         Zv[c] = (j >= 0 && j < m) ? (x[j] - mu) / sg : -kInf;
     }
-    int jb = 0;
-    float zb = lane < m ? (x[lane] - mu) / sg : -kInf;  // z[jb + lane]
-    // Query values enter at the top slot, q[(k - W) / 2 + 64C] on even steps;
-    // they are fetched 64 at a time into qb (q[ib + lane]) and taken with
-    // readlane, so no step waits on a load.
-    int ib = -kWave;
-    float qb = kInf;
+    // Index of the query value entering at the next parity-0 step and of the
+    // stream value entering at the next parity-1 step (diagonal k: (k - W)/2
+    // + 64C and (k + W + 1)/2).
+    int iq = (FIRST == 0 ? 0 : 1) - W;
+    iq = iq / 2 + kWave * C;
+    int iz = (FIRST == 1 ? 0 : 1) + W + 1;
+    iz = iz / 2;
+    int ib = iq & ~(kWave - 1);
+    float qb = ib + lane < m ? q[ib + lane] : kInf;
+    int jb = iz & ~(kWave - 1);
+    float zb = jb + lane < m ? (x[jb + lane] - mu) / sg : -kInf;
+    const bool top = lane == kWave - 1, bottom = lane == 0;
+
+    auto shift_q = [&]() {  // i advances: slot p takes slot p+1's query value
+        const float in = lane_from_above(Qv[0], kInf);
+#pragma unroll
+        for (int c = 0; c + 1 < C; ++c) Qv[c] = Qv[c + 1];
+        if (iq >= ib + kWave) {
+            ib = iq & ~(kWave - 1);
+            qb = ib + lane < m ? q[ib + lane] : kInf;
+        }
+        const float fresh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), iq - ib));
+        Qv[C - 1] = top ? fresh : in;
+        ++iq;
+    };
+    auto shift_z = [&]() {  // j advances: slot p takes slot p-1's stream value
+        const float in = lane_from_below(Zv[C - 1], -kInf);
+#pragma unroll
+        for (int c = C - 1; c > 0; --c) Zv[c] = Zv[c - 1];
+        if (iz >= jb + kWave) {
+            jb = iz & ~(kWave - 1);
+            zb = jb + lane < m ? (x[jb + lane] - mu) / sg : -kInf;
+        }
+        const float fresh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zb), iz - jb));
+        Zv[0] = bottom ? fresh : in;
+        ++iz;
+    };
+    auto step_first = [&]() {  // diagonal 2p (parity FIRST) into X
+        if constexpr (FIRST == 0) {
+            wide_step<C, L2, 0>(X, Y, Qv, Zv, pad0);
+            shift_q();
+        } else {
+            wide_step<C, L2, 1>(X, Y, Qv, Zv, pad1);
+            shift_z();
+        }
+    };
+    auto step_second = [&]() {  // diagonal 2p + 1 into Y
+        if constexpr (FIRST == 0) {
+            wide_step<C, L2, 1>(Y, X, Qv, Zv, pad1);
+            shift_z();
+        } else {
+            wide_step<C, L2, 0>(Y, X, Qv, Zv, pad0);
+            shift_q();
+        }
+    };
+
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     const float thr = abandon ? load_abandon_bound(ctrl, mode) : kInf;
-    const int K = 2 * m - 1;
-    for (int k = 0; k < K; ++k) {
-        const int par = (k + W) & 1;
-        float nw[C];
-        if (par == 0) {
-            const float in = lane_from_below(A1[C - 1], kInf);
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const float up = c > 0 ? A1[c - 1] : in;
-                const float mn = fminf(fminf(A2[c], up), A1[c]);
-                const float dd = Qv[c] - Zv[c];
-                const float v = L2 ? fmaf(dd, dd, mn) : fabsf(dd) + mn;
-                nw[c] = fmaxf(v, pad0[c]);
-            }
-        } else {
-            const float in = lane_from_above(A1[0], kInf);
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const float lf = c + 1 < C ? A1[c + 1] : in;
-                const float mn = fminf(fminf(A2[c], A1[c]), lf);
-                const float dd = Qv[c] - Zv[c];
-                const float v = L2 ? fmaf(dd, dd, mn) : fabsf(dd) + mn;
-                nw[c] = fmaxf(v, pad1[c]);
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-            A2[c] = A1[c];
-            A1[c] = nw[c];
-        }
-        if (par == 0) {
-            // i advances: slot p takes slot p+1's query value.
-            const float in = lane_from_above(Qv[0], kInf);
-#pragma unroll
-            for (int c = 0; c + 1 < C; ++c) Qv[c] = Qv[c + 1];
-            const int iq = (k - W) / 2 + kWave * C;
-            float fresh = kInf;
-            if (iq >= 0 && iq < m) {
-                if (iq >= ib + kWave) {
-                    ib = iq & ~(kWave - 1);
-                    qb = ib + lane < m ? q[ib + lane] : kInf;
-                }
-                fresh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(qb), iq - ib));
-            }
-            Qv[C - 1] = lane == kWave - 1 ? fresh : in;
-        } else {
-            // j advances: slot p takes slot p-1's stream value.
-            const float in = lane_from_below(Zv[C - 1], -kInf);
-#pragma unroll
-            for (int c = C - 1; c > 0; --c) Zv[c] = Zv[c - 1];
-            const int iz = (k + W + 1) / 2;
-            float fresh = -kInf;
-            if (iz < m) {
-                if (iz >= jb + kWave) {
-                    jb = iz & ~(kWave - 1);
-                    zb = jb + lane < m ? (x[jb + lane] - mu) / sg : -kInf;
-                }
-                fresh = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(zb), iz - jb));
-            }
-            Zv[0] = lane == 0 ? fresh : in;
-        }
-        if (abandon && (k % kWideCheck) == kWideCheck - 1) {
+    // 2M - 1 diagonals: M - 1 pairs, then diagonal 2M - 2 alone.
+    for (int32_t pr = 0; pr + 1 < m; ++pr) {
+        step_first();
+        step_second();
+        if (abandon && (pr % (kWideCheck / 2)) == kWideCheck / 2 - 1) {
             float cm = kInf;
 #pragma unroll
-            for (int c = 0; c < C; ++c) cm = fminf(cm, fminf(A1[c], A2[c]));
+            for (int c = 0; c < C; ++c) cm = fminf(cm, fminf(X[c], Y[c]));
             cm = wave_min_f32(cm);
             if (cm > thr) {
                 alive = false;
@@ -155,12 +186,21 @@ __device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
             }
         }
     }
-    // D[M-1][M-1]: diagonal 2M-2, d = 0 -> slot W >> 1.
-    float r = A1[0];
+    step_first();
+    // D[M-1][M-1]: diagonal 2M-2 (in X), d = 0 -> slot W >> 1.
+    float r = X[0];
 #pragma unroll
     for (int c = 1; c < C; ++c)
-        if (c == pfin % C) r = A1[c];
+        if (c == pfin % C) r = X[c];
     return __shfl(r, pfin / C, kWave);
+}
+
+template <int C, bool L2>
+__device__ float wide_dp(const float* __restrict__ x, float mu, float sg,
+                         const float* __restrict__ q, int32_t m, int32_t W, int mode,
+                         const Ctrl* ctrl, bool& alive) {
+    return (W & 1) ? wide_dp_pairs<C, L2, 1>(x, mu, sg, q, m, W, mode, ctrl, alive)
+                   : wide_dp_pairs<C, L2, 0>(x, mu, sg, q, m, W, mode, ctrl, alive);
 }
 
 template <int C, bool L2>
