@@ -42,28 +42,37 @@ constexpr int kBandWaves = 4;  // waves per workgroup
 constexpr int kBandSlideMax = 54;
 // Bands up to kBandMaxW unroll kBandU columns per block (W = 108..110: two,
 // W = 111..112: one, which keeps them within 256 registers, i.e. two waves
-// per SIMD); wider bands (up to kBandWideMaxW) unroll one column, which
-// keeps the 2W + 1 band values in vector + accumulation registers (no scratch
-// memory): two waves per SIMD up to W = 114, one from W = 137; W = 115..136
-// keep part of the band in LDS instead (below) at two waves per SIMD.
+// per SIMD); wider bands (up to kBandWideMaxW) keep the 2W + 1 band values
+// in vector registers (two waves per SIMD up to W = 114, one column at a
+// time), in vector registers plus an LDS tail (two waves per SIMD, below), or
+// in vector + accumulation registers at one wave per SIMD, two columns at a
+// time in a skewed pair (band_skewed).
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
-// Bands W = 115..kBandTailMaxW keep band values 0..kBandTailR-1 in vector
-// registers and the rest (the tail, t >= kBandTailR) in LDS, 16 bytes per
-// lane per chunk of four values: at most 16 chunks, 64 KiB per workgroup, so
-// two workgroups (two waves per SIMD, the register budget of 256) share a
-// CU's 160 KiB.  Without the tail these bands need accumulation registers
-// and run at one wave per SIMD, where each column's serial min3 -> fmac
-// chain is exposed (half the rate).  Their kernels are compiled for two
+// Bands W = 115..kBandTailMaxW (L2; L1 to kBandTailMaxWL1) keep band values
+// 0..kBandTailR-1 in vector registers and the rest (the tail, t >= kBandTailR)
+// in LDS, 16 bytes per lane per chunk of four values: at most 16 chunks,
+// 64 KiB per workgroup, so two workgroups (two waves per SIMD, the register
+// budget of 256) share a CU's 160 KiB.  Without the tail these bands need
+// accumulation registers and run at one wave per SIMD, where each column's
+// serial min3 -> fmac chain is exposed.  Their kernels are compiled for two
 // waves per SIMD (band_kernel_tail); the compiler then spills a few band
 // values per column to scratch memory, a number that grows with the tail:
-// measured at M = 1024, the tail kernel runs W = 115 at 18 T cells/s (one
-// wave per SIMD without it: 10), 120 at 16 (9), 128 at 12 (9), 136 at 10
-// (9) and 143 at 8 (9), hence the limit.
+// measured at M = 1024, the L2 tail kernel runs W = 115 at 18 T cells/s (one
+// wave per SIMD, one column at a time: 10), 120 at 16, 128 at 12 and 136 at
+// 10, while the one-wave skewed pairs below run 128..136 at 11; the L1 tail
+// kernel falls to 9 at W = 130 and 7 at 134 (skewed pairs: 11), hence the
+// two limits.
 constexpr int kBandTailR = 212;
 constexpr int kBandTailMaxW = 136;
-__host__ __device__ constexpr int band_tail_len(int w) {
-    return (w > 114 && w <= kBandTailMaxW) ? 2 * w + 1 - kBandTailR : 0;
+constexpr int kBandTailMaxWL1 = 129;
+__host__ __device__ constexpr int band_tail_len(int w, bool l2) {
+    return (w > 114 && w <= (l2 ? kBandTailMaxW : kBandTailMaxWL1)) ? 2 * w + 1 - kBandTailR : 0;
+}
+// Bands beyond the tail take their columns in skewed pairs (band_dp).
+template <int W, bool L2>
+constexpr bool band_skewed() {
+    return band_tail_len(W, L2) == 0 && W > 114;
 }
 constexpr int kBandTailStride = 4 * 64;  // float4 slots between a lane's chunks
 
@@ -110,6 +119,16 @@ __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t first
     return static_cast<int64_t>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// d * d + mn with one rounding, as fmaf, but always the three-address
+// v_fma_f32: in the skewed pairs the compiler's choice, v_fmac_f32 (whose
+// destination is its addend), left it spilling 100-230 bytes per lane to
+// scratch memory at W = 216..236 (L2), where this form spills none to 48.
+__device__ __forceinline__ float band_fma_sq(float d, float mn) {
+    float e;
+    asm("v_fma_f32 %0, %1, %1, %2" : "=v"(e) : "v"(d), "v"(mn));
+    return e;
+}
+
 // One column of the band update with the query values of this column in
 // q[0..B) (wave-uniform scalars).
 template <int B, bool L2, class Q>
@@ -137,7 +156,10 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                                          const float* __restrict__ elo,
                                          const float* __restrict__ eup, float lbt) {
     constexpr int B = 2 * W + 1;
-    constexpr int U = W > 110 ? 1 : (W > 107 ? 2 : kBandU);
+    // Bands wider than the LDS tail's range run one wave per SIMD and take
+    // their columns in skewed pairs (kSkew).
+    constexpr bool kSkew = band_skewed<W, L2>();
+    constexpr int U = kSkew ? 2 : (W > 110 ? 1 : (W > 107 ? 2 : kBandU));
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
     // Narrow bands keep the query window of a whole block of U columns in
@@ -148,7 +170,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     // query in chunks of CH scalars per column.
     constexpr bool kSlide = B + 2 * U - 1 <= kBandSlideMax;
     // KT band values in the LDS tail (0 for most bands), BR in registers.
-    constexpr int KT = band_tail_len(W);
+    constexpr int KT = band_tail_len(W, L2);
     constexpr int KT4 = (KT + 3) / 4;
     constexpr int BR = B - KT;
     static_assert(KT == 0 || (U == 1 && !kSlide && W < BR), "tail layout");
@@ -160,6 +182,8 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     // minimum of the tail's values after the last column (for the abandon
     // check, which then needs no LDS reads)
     [[maybe_unused]] float tail_min = kInf;
+    // skewed pairs: E[W] of the pair's first column (the answer for odd M)
+    [[maybe_unused]] float ew0 = kInf;
     const float4 inf4 = make_float4(kInf, kInf, kInf, kInf);
     if constexpr (KT4 > 0) {
         tl = band_tail_lds<KT4>();
@@ -307,6 +331,68 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
 #pragma unroll
                 for (int i = 0; i < CH; ++i) qa[i] = qb[i];
             }
+        } else if constexpr (kSkew) {
+            // Two columns per step (U == 2), the second two cells behind the
+            // first.  Cell t of column j0+1 needs cells t and t+1 of column
+            // j0, written one and two steps earlier, so the two dependency
+            // chains of a step are independent: the instruction-level
+            // parallelism a band this wide (one wave per SIMD) lacks with one
+            // column.  Both update E in place: column j0 overwrites cell s
+            // with its new value, column j0+1 cell s-2.  Column j0+1 reads
+            // the query one row behind column j0 (qc[s-1] at step s).  For
+            // odd M the last pair's second column is a dummy (its sample
+            // index clamped): the answer is then column M-1's E[W], kept in
+            // ew0, and that block skips the abandon check.  (A separate
+            // one-column path for the last block doubled the loop's code and
+            // sent the band to scratch memory.)
+            const float* __restrict__ qc = qx + j0;
+            constexpr int NS = (B + 2 + CH - 1) / CH;
+            const float z0 = z[0], z1 = z[1];
+            float p0 = kInf, p1 = kInf, ql = kInf;
+            float qa[CH], qb[CH];
+            auto step = [&](int s, int i) {
+                if (s < B) {
+                    const float nxt = s + 1 < B ? E[s + 1] : kInf;
+                    const float mn = fminf(fminf(p0, E[s]), nxt);
+                    const float d = qa[i] - z0;
+                    const float e = L2 ? band_fma_sq(d, mn) : fabsf(d) + mn;
+                    E[s] = e;
+                    p0 = e;
+                    if (s == W) ew0 = e;
+                }
+                if (s >= 2 && s - 2 < B) {
+                    const int t = s - 2;
+                    const float nxt = t + 1 < B ? E[t + 1] : kInf;
+                    const float mn = fminf(fminf(p1, E[t]), nxt);
+                    const float d = (i > 0 ? qa[i - 1] : ql) - z1;
+                    const float e = L2 ? band_fma_sq(d, mn) : fabsf(d) + mn;
+                    E[t] = e;
+                    p1 = e;
+                }
+            };
+#pragma unroll
+            for (int i = 0; i < CH; ++i) qa[i] = qc[i];
+            // Scalar loads complete out of order, so every wait is for all of
+            // them: the next chunk is requested after the first step of this
+            // one (whose wait is for this chunk only) and a chunk ahead of its
+            // use.  The barriers keep that order (left free, the scheduler
+            // sinks the loads to just before their use, and one wave per SIMD
+            // has nothing to hide the wait).
+#pragma unroll
+            for (int ch = 0; ch < NS; ++ch) {
+                step(ch * CH, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (ch + 1 < NS) {
+#pragma unroll
+                    for (int i = 0; i < CH; ++i) qb[i] = qc[(ch + 1) * CH + i];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 1; i < CH; ++i) step(ch * CH + i, i);
+                ql = qa[CH - 1];
+#pragma unroll
+                for (int i = 0; i < CH; ++i) qa[i] = qb[i];
+            }
         } else {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -345,7 +431,8 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                 }
             }
         }
-        if (abandon && ((j0 / U) % (kBandCheck / U)) == (kBandCheck / U) - 1) {
+        if (abandon && ((j0 / U) % (kBandCheck / U)) == (kBandCheck / U) - 1 &&
+            (!kSkew || j0 + U <= m)) {
             float cm = E[0];
 #pragma unroll
             for (int t = 1; t < BR; ++t) cm = fminf(cm, E[t]);
@@ -363,7 +450,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
             if (!__any(alive)) return kInf;
         }
     }
-    return E[W];
+    return kSkew && (m & 1) ? ew0 : E[W];
 }
 
 __device__ __forceinline__ uint32_t band_lane_rank(unsigned long long mask) {
@@ -466,7 +553,7 @@ __global__ __launch_bounds__(256) void band_kernel(MDTW_BAND_KERNEL_ARGS) {
     band_kernel_body<W, L2, LB>(s, avg, sd, qx, idx, out, res, ctrl, a, lbt, elo, eup);
 }
 
-// Bands with an LDS tail, compiled for two waves per SIMD (see kBandTailMaxW).
+// Bands with an LDS tail, compiled for two waves per SIMD (see band_tail_len).
 template <int W, bool L2, bool LB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void band_kernel_tail(
     MDTW_BAND_KERNEL_ARGS) {
@@ -476,7 +563,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 template <int W, bool L2, bool LB>
 constexpr auto band_kernel_fn() {
-    if constexpr (band_tail_len(W) > 0)
+    if constexpr (band_tail_len(W, L2) > 0)
         return &band_kernel_tail<W, L2, LB>;
     else
         return &band_kernel<W, L2, LB>;
