@@ -46,7 +46,7 @@ constexpr int kBandSlideMax = 54;
 // in vector registers (two waves per SIMD up to W = 114, one column at a
 // time), in vector registers plus an LDS tail (two waves per SIMD, below), or
 // in vector + accumulation registers at one wave per SIMD, two columns at a
-// time in a skewed pair (band_skewed).
+// time, skewed (band_skewed).
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 // Bands W = 115..kBandTailMaxW (L2; L1 to kBandTailMaxWL1) keep band values
@@ -60,8 +60,8 @@ constexpr int kBandWideMaxW = 240;
 // values per column to scratch memory, a number that grows with the tail:
 // measured at M = 1024, the L2 tail kernel runs W = 115 at 18 T cells/s (one
 // wave per SIMD, one column at a time: 10), 120 at 16, 128 at 12 and 136 at
-// 10, while the one-wave skewed pairs below run 128..136 at 11; the L1 tail
-// kernel falls to 9 at W = 130 and 7 at 134 (skewed pairs: 11), hence the
+// 10, while the one-wave skewed columns below run 128..136 at 11; the L1 tail
+// kernel falls to 9 at W = 130 and 7 at 134 (skewed columns: 11), hence the
 // two limits.
 constexpr int kBandTailR = 212;
 constexpr int kBandTailMaxW = 136;
@@ -69,7 +69,10 @@ constexpr int kBandTailMaxWL1 = 129;
 __host__ __device__ constexpr int band_tail_len(int w, bool l2) {
     return (w > 114 && w <= (l2 ? kBandTailMaxW : kBandTailMaxWL1)) ? 2 * w + 1 - kBandTailR : 0;
 }
-// Bands beyond the tail take their columns in skewed pairs (band_dp).
+// Bands beyond the tail take their columns U at a time, skewed (band_dp):
+// four up to kBandSkew4MaxW, two beyond, where four made the compiler spill
+// the band to scratch memory.
+constexpr int kBandSkew4MaxW = 232;
 template <int W, bool L2>
 constexpr bool band_skewed() {
     return band_tail_len(W, L2) == 0 && W > 114;
@@ -120,9 +123,10 @@ __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t first
 }
 
 // d * d + mn with one rounding, as fmaf, but always the three-address
-// v_fma_f32: in the skewed pairs the compiler's choice, v_fmac_f32 (whose
-// destination is its addend), left it spilling 100-230 bytes per lane to
-// scratch memory at W = 216..236 (L2), where this form spills none to 48.
+// v_fma_f32: in the skewed pairs (two columns) the compiler's choice,
+// v_fmac_f32 (whose destination is its addend), left it spilling 100-230
+// bytes per lane to scratch memory at W = 216..236 (L2), where this form
+// spills none to 64.  (With four columns it is the other way round.)
 __device__ __forceinline__ float band_fma_sq(float d, float mn) {
     float e;
     asm("v_fma_f32 %0, %1, %1, %2" : "=v"(e) : "v"(d), "v"(mn));
@@ -157,9 +161,12 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                                          const float* __restrict__ eup, float lbt) {
     constexpr int B = 2 * W + 1;
     // Bands wider than the LDS tail's range run one wave per SIMD and take
-    // their columns in skewed pairs (kSkew).
+    // their columns in skewed groups (kSkew).
     constexpr bool kSkew = band_skewed<W, L2>();
-    constexpr int U = kSkew ? 2 : (W > 110 ? 1 : (W > 107 ? 2 : kBandU));
+    constexpr int U =
+        kSkew ? (W <= kBandSkew4MaxW ? 4 : 2) : (W > 110 ? 1 : (W > 107 ? 2 : kBandU));
+    // L2 cell of two skewed columns: three-address fma (see band_fma_sq)
+    constexpr bool kFma3 = kSkew && U == 2;
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
     // Narrow bands keep the query window of a whole block of U columns in
@@ -182,8 +189,11 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     // minimum of the tail's values after the last column (for the abandon
     // check, which then needs no LDS reads)
     [[maybe_unused]] float tail_min = kInf;
-    // skewed pairs: E[W] of the pair's first column (the answer for odd M)
-    [[maybe_unused]] float ew0 = kInf;
+    // skewed columns: E[W] of each but the last column of a block (the
+    // answer when M is not a multiple of U)
+    [[maybe_unused]] float ew[U > 1 ? U - 1 : 1];
+#pragma unroll
+    for (int u = 0; u + 1 < U; ++u) ew[u] = kInf;
     const float4 inf4 = make_float4(kInf, kInf, kInf, kInf);
     if constexpr (KT4 > 0) {
         tl = band_tail_lds<KT4>();
@@ -332,42 +342,42 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                 for (int i = 0; i < CH; ++i) qa[i] = qb[i];
             }
         } else if constexpr (kSkew) {
-            // Two columns per step (U == 2), the second two cells behind the
-            // first.  Cell t of column j0+1 needs cells t and t+1 of column
-            // j0, written one and two steps earlier, so the two dependency
-            // chains of a step are independent: the instruction-level
-            // parallelism a band this wide (one wave per SIMD) lacks with one
-            // column.  Both update E in place: column j0 overwrites cell s
-            // with its new value, column j0+1 cell s-2.  Column j0+1 reads
-            // the query one row behind column j0 (qc[s-1] at step s).  For
-            // odd M the last pair's second column is a dummy (its sample
-            // index clamped): the answer is then column M-1's E[W], kept in
-            // ew0, and that block skips the abandon check.  (A separate
-            // one-column path for the last block doubled the loop's code and
-            // sent the band to scratch memory.)
+            // U columns per step, each two cells behind the one before.  Cell
+            // t of column j0+u needs cells t and t+1 of column j0+u-1,
+            // written one and two steps earlier, so the U dependency chains
+            // of a step are independent: the instruction-level parallelism a
+            // band this wide (one wave per SIMD) lacks with one column.  All
+            // update E in place: column j0+u overwrites cell s-2u at step s,
+            // reading the query u rows behind column j0 (qc[s-u]).  When M is
+            // not a multiple of U the last block's trailing columns are
+            // dummies (their sample index clamped): the answer is then column
+            // M-1's E[W], kept in ew, and that block skips the abandon check.
+            // (A separate path for a partial last block doubled the loop's
+            // code and sent the band to scratch memory.)
             const float* __restrict__ qc = qx + j0;
-            constexpr int NS = (B + 2 + CH - 1) / CH;
-            const float z0 = z[0], z1 = z[1];
-            float p0 = kInf, p1 = kInf, ql = kInf;
+            constexpr int NS = (B + 2 * (U - 1) + CH - 1) / CH;
+            float p[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) p[u] = kInf;
+            // ql[k] = qc[ch * CH - 1 - k]: the previous chunk's last values
+            float ql[U - 1];
+#pragma unroll
+            for (int k = 0; k + 1 < U; ++k) ql[k] = kInf;
             float qa[CH], qb[CH];
             auto step = [&](int s, int i) {
-                if (s < B) {
-                    const float nxt = s + 1 < B ? E[s + 1] : kInf;
-                    const float mn = fminf(fminf(p0, E[s]), nxt);
-                    const float d = qa[i] - z0;
-                    const float e = L2 ? band_fma_sq(d, mn) : fabsf(d) + mn;
-                    E[s] = e;
-                    p0 = e;
-                    if (s == W) ew0 = e;
-                }
-                if (s >= 2 && s - 2 < B) {
-                    const int t = s - 2;
-                    const float nxt = t + 1 < B ? E[t + 1] : kInf;
-                    const float mn = fminf(fminf(p1, E[t]), nxt);
-                    const float d = (i > 0 ? qa[i - 1] : ql) - z1;
-                    const float e = L2 ? band_fma_sq(d, mn) : fabsf(d) + mn;
-                    E[t] = e;
-                    p1 = e;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = s - 2 * u;
+                    if (t >= 0 && t < B) {
+                        const float nxt = t + 1 < B ? E[t + 1] : kInf;
+                        const float mn = fminf(fminf(p[u], E[t]), nxt);
+                        const float d = (i >= u ? qa[i - u] : ql[u - i - 1]) - z[u];
+                        const float e = L2 ? (kFma3 ? band_fma_sq(d, mn) : fmaf(d, d, mn))
+                                           : fabsf(d) + mn;
+                        E[t] = e;
+                        p[u] = e;
+                        if (u + 1 < U && t == W) ew[u] = e;
+                    }
                 }
             };
 #pragma unroll
@@ -389,7 +399,8 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int i = 1; i < CH; ++i) step(ch * CH + i, i);
-                ql = qa[CH - 1];
+#pragma unroll
+                for (int k = 0; k + 1 < U; ++k) ql[k] = qa[CH - 1 - k];
 #pragma unroll
                 for (int i = 0; i < CH; ++i) qa[i] = qb[i];
             }
@@ -450,7 +461,15 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
             if (!__any(alive)) return kInf;
         }
     }
-    return kSkew && (m & 1) ? ew0 : E[W];
+    if constexpr (kSkew) {
+        const int last = (m - 1) % U;  // column M-1's place in its block
+        float r = E[W];
+#pragma unroll
+        for (int u = 0; u + 1 < U; ++u)
+            if (last == u) r = ew[u];
+        return r;
+    }
+    return E[W];
 }
 
 __device__ __forceinline__ uint32_t band_lane_rank(unsigned long long mask) {

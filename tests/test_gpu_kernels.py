@@ -91,7 +91,7 @@ def test_stats_refcompat_matches_restatement(engine):
                                  (400, 137), (300, 144), (400, 145), (400, 177), (500, 208),
                                  (500, 209), (600, 240), (700, 241), (401, 137), (333, 150),
                                  (501, 201), (601, 240), (151, 160), (3, 150), (333, 215),
-                                 (334, 216)])
+                                 (334, 216), (335, 232), (337, 233), (3, 233), (2, 200)])
 def test_cdtw_bit_exact_random_order(engine, m, w):
     n = 6000
     s = synth.random_walk(n, m + w)
@@ -120,7 +120,7 @@ def test_cdtw_wide_kernel_bit_exact(engine, w):
 
 @pytest.mark.parametrize("m,w", [(300, 112), (300, 113), (300, 115), (400, 136), (400, 137),
                                  (500, 200), (600, 240), (700, 241), (401, 151), (601, 239), (300, 129),
-                                 (300, 130), (401, 131)])
+                                 (300, 130), (401, 131), (402, 140), (403, 200)])
 def test_cdtw_l1_wide_bands_bit_exact(engine, m, w):
     # L1 bands above 112: register band (LDS tail at 115..129, skewed column
     # pairs from 130) to W = 240,
