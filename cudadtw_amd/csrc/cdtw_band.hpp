@@ -45,7 +45,7 @@ constexpr int kBandSlideMax = 54;
 // per SIMD); wider bands (up to kBandWideMaxW) keep the 2W + 1 band values
 // in vector registers (two waves per SIMD up to W = 114, one column at a
 // time), in vector registers plus an LDS tail (two waves per SIMD, below), or
-// in vector + accumulation registers at one wave per SIMD, two columns at a
+// in vector + accumulation registers at one wave per SIMD, four or two columns at a
 // time, skewed (band_skewed).
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
