@@ -183,8 +183,9 @@ def section_c2(eng, torch, args):
     log(f"[c2] memory-side kernels over all windows: {mem}")
 
     # DP rate on wide bands (|Q| = 1024 at 10 / 15.6 / 20 %: W = 102 in the
-    # four-column register kernel, 160 and 205 in the one-wave-per-SIMD wide
-    # register kernel), profile mode over a 2^24-sample prefix of the stream.
+    # four-column register kernel, 160 and 205 in the one-wave-per-SIMD
+    # register kernel that takes four skewed columns per step), profile mode
+    # over a 2^24-sample prefix of the stream.
     wide = {}
     n24 = min(n, 1 << 24)
     q1k = znormalize(synth.cut_query(stream[:n24], 1024, 11)[0])
