@@ -21,6 +21,7 @@ namespace {
 
 constexpr float kLdtwInf = 1e10f;  // ldtw.cuh:8 INFTY
 constexpr int kLdtwCheck = 32;
+constexpr int kLdtwLdsMaxM = 2048;
 
 template <bool L2>
 __device__ __forceinline__ float lcost(float t) {
@@ -31,7 +32,7 @@ __device__ __forceinline__ float lcost(float t) {
 }
 
 template <bool L2>
-__device__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
+__device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
                              const float* __restrict__ q, int32_t M, int32_t W, int mode,
                              const Ctrl* ctrl, bool& alive) {
     int32_t i = 0, j = 0;
@@ -93,12 +94,22 @@ __device__ __forceinline__ uint32_t rank_in(unsigned long long mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
 }
 
-template <bool L2>
+// QL: the query is staged in LDS (dynamic shared memory, 4M bytes): the
+// path's per-lane query reads are gathers, and from global memory they
+// queue in the vector-memory address unit behind the stream reads.
+template <bool L2, bool QL>
 __global__ __launch_bounds__(64) void ldtw_kernel(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
-    const float* __restrict__ q, const int32_t* __restrict__ idx, float* __restrict__ out,
+    const float* __restrict__ qg, const int32_t* __restrict__ idx, float* __restrict__ out,
     unsigned long long* __restrict__ res, Ctrl* __restrict__ ctrl, BandScalars a, int32_t W) {
     const int lane = threadIdx.x;
+    extern __shared__ float ldtw_q[];
+    const float* __restrict__ q = qg;
+    if constexpr (QL) {
+        for (int k = lane; k < a.m; k += kWave) ldtw_q[k] = qg[k];
+        __syncthreads();
+        q = ldtw_q;
+    }
     int64_t count = a.count;
     if (a.idx_count_from_ctrl)
         count = static_cast<int64_t>(
@@ -153,12 +164,14 @@ int launch_ldtw(hipStream_t st, int cus, const BandArgs& a, int32_t w, int32_t m
     BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent};
     const unsigned grid = a.persistent ? static_cast<unsigned>(cus * 16)
                                        : static_cast<unsigned>((a.count + kWave - 1) / kWave);
-    if (metric)
-        hipLaunchKernelGGL(ldtw_kernel<true>, dim3(grid), dim3(kWave), 0, st, a.s, a.avg, a.sd,
-                           a.qx, a.idx, a.out, a.res, a.ctrl, sc, w);
-    else
-        hipLaunchKernelGGL(ldtw_kernel<false>, dim3(grid), dim3(kWave), 0, st, a.s, a.avg, a.sd,
-                           a.qx, a.idx, a.out, a.res, a.ctrl, sc, w);
+    // Query in LDS up to kLdtwLdsMaxM (8 KiB per one-wave workgroup keeps
+    // LDS from limiting residency); longer queries read it from global memory.
+    const bool ql = a.m <= kLdtwLdsMaxM;
+    const size_t lds = ql ? sizeof(float) * static_cast<size_t>(a.m) : 0;
+    auto k = metric ? (ql ? ldtw_kernel<true, true> : ldtw_kernel<true, false>)
+                    : (ql ? ldtw_kernel<false, true> : ldtw_kernel<false, false>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kWave), lds, st, a.s, a.avg, a.sd, a.qx, a.idx, a.out,
+                       a.res, a.ctrl, sc, w);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);
