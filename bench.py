@@ -355,38 +355,53 @@ def section_c4(eng, torch, args):
     k = 8
     Ls = np.array([s.size - z.size + 1 for z in zqs], np.float64)
     cellsum = float(sum(L * cells(z.size, int(b)) for L, z, b in zip(Ls, zqs, bs)))
-    batch = eng.batch(zqs, bs, pipelines=4)
-    t0 = time.perf_counter()
-    nb = batch.tune(ds, k=k, cell_budget=1 << 38)
-    t_tune = time.perf_counter() - t0
-    log(f"[c4] tuned {nb} buckets in {t_tune:.1f} s; capturing and running the batch ...")
-    rb = batch.search(ds, k=k)  # captures the graph
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    rb = batch.search(ds, k=k)
-    t_batch = time.perf_counter() - t0
-    log(f"[c4] batch of {len(zqs)}: {t_batch:.3f} s (graph device time {rb.elapsed_ms:.1f} ms)")
-    pos = np.full((len(zqs), k), -1, np.int64)
-    dist = np.full((len(zqs), k), np.inf, np.float32)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i, (z, w) in enumerate(zip(zqs, bs)):
-        eng.set_query(z, int(w))
-        r = eng.search(ds, int(w), k=k, excl=z.size // 2)
-        pos[i], dist[i] = r.positions, r.distances
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    same = bool(np.array_equal(pos, rb.positions) and np.array_equal(dist, rb.distances))
-    log(f"[c4] plain, one search per query: {t_plain:.3f} s; speedup {t_plain / t_batch:.2f}x; "
-        f"identical results: {same}")
-    batch.close()
+    # One batch over the selected queries, or (--c4-split G) consecutive
+    # groups of G queries, one batch each (progress per group for the full
+    # set, whose single launch runs for minutes).  Per group: tuning and the
+    # capturing first search untimed, the second search timed; then one
+    # plain search per query of the group, timed.
+    G = args.c4_split if args.c4_split > 0 else len(zqs)
+    t_batch = t_plain = t_tune = 0.0
+    nb = 0
+    same = True
+    for g0 in range(0, len(zqs), G):
+        gz, gb = zqs[g0:g0 + G], bs[g0:g0 + G]
+        batch = eng.batch(gz, gb, pipelines=4)
+        t0 = time.perf_counter()
+        nb += batch.tune(ds, k=k, cell_budget=1 << 38)
+        t_tune += time.perf_counter() - t0
+        rb = batch.search(ds, k=k)  # captures the graph
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rb = batch.search(ds, k=k)
+        tb = time.perf_counter() - t0
+        pos = np.full((len(gz), k), -1, np.int64)
+        dist = np.full((len(gz), k), np.inf, np.float32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i, (z, w) in enumerate(zip(gz, gb)):
+            eng.set_query(z, int(w))
+            r = eng.search(ds, int(w), k=k, excl=z.size // 2)
+            pos[i], dist[i] = r.positions, r.distances
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        gsame = bool(np.array_equal(pos, rb.positions) and np.array_equal(dist, rb.distances))
+        batch.close()
+        t_batch += tb
+        t_plain += tp
+        same = same and gsame
+        log(f"[c4] queries {g0}..{g0 + len(gz) - 1}: batch {tb:.3f} s (graph device time "
+            f"{rb.elapsed_ms:.1f} ms), plain {tp:.3f} s, identical: {gsame}")
+    log(f"[c4] {len(zqs)} queries in batches of {G}: batch {t_batch:.3f} s, plain "
+        f"{t_plain:.3f} s; speedup {t_plain / t_batch:.2f}x; identical results: {same}")
 
     short = c4_short(eng, torch, k)
     return {"workload": f"ECG-like 2^26, {len(zqs)} of 1024 queries "
                         f"({'all' if args.c4 == 'full' else 'every %d-th' % args.c4_stride}), "
                         "|Q| 64-1024, bands 5-20%, half cut+N(0,0.05), half walks, top-8, "
                         "excl |Q|/2",
-            "queries": len(zqs), "wide_band_queries": int(np.sum(bs > 112)),
+            "queries": len(zqs), "queries_per_batch": G,
+            "wide_band_queries": int(np.sum(bs > 112)),
             "batch_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
             "speedup": round(t_plain / t_batch, 3), "identical": same,
             "tune_s": round(t_tune, 2), "buckets_tuned": nb,
@@ -494,6 +509,8 @@ def main():
     ap.add_argument("--sections", default="c2,c1,c3,c4,c5")
     ap.add_argument("--c4", choices=["sample", "full", "short", "off"], default="sample")
     ap.add_argument("--c4-stride", type=int, default=32)
+    ap.add_argument("--c4-split", type=int, default=0,
+                    help="c4: batches of this many queries (0: one batch)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
