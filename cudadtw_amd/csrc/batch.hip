@@ -225,13 +225,12 @@ int capture(mdtw_batch* b, const float* s, int64_t n, int32_t metric, int32_t k,
     std::vector<int32_t> order(b->nq);
     std::iota(order.begin(), order.end(), 0);
     std::vector<double> cost(b->nq);
-    // Relative cell rates of the kernels a band lands on (register band with
-    // four- or one-column unroll, anti-diagonal).
+    // Relative time per cell of the kernels a band lands on (register band
+    // at two or more waves per SIMD, ~18 T cells/s; wider register bands,
+    // LDS tail or skewed columns at one wave per SIMD, ~9-16; anti-diagonal).
     for (int32_t i = 0; i < b->nq; ++i) {
         const int32_t w = b->w[i];
-        const double rate = w <= kBandMaxW ? 1.0
-                            : w <= kBandWideMaxW ? 2.2
-                                                                             : 6.5;
+        const double rate = w <= kBandMaxW ? 1.0 : w <= kBandWideMaxW ? 1.7 : 6.5;
         cost[i] = static_cast<double>(n - b->m[i] + 1) * band_cells(b->m[i], w) * rate;
     }
     std::stable_sort(order.begin(), order.end(),
