@@ -515,8 +515,8 @@ def main():
                     help="also run one 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every GPU box here has one GPU, and the "
-              "search path has no multi-GPU code (see DESIGN.md for how it would shard).",
+        print("bench.py measures one GPU only: the search path has no multi-GPU code "
+              "(it would shard by window start, one stream slice plus an M-1 halo per GPU).",
               file=sys.stderr)
         sys.exit(2)
     sections = set(args.sections.split(","))

@@ -74,7 +74,7 @@ def test_fft_matches_direct_ed_profile(engine):
     direct = host(engine.ed(ds, avg, sd)).astype(np.float64)
     fft = host(engine.ed_fft(ds)).astype(np.float64)
     want = oracle_profile(s, zq)
-    # both within the north-star 1e-5 of FP64 (direct: FP32 arithmetic)
+    # both within 1e-5 relative of FP64 (direct: FP32 arithmetic)
     assert np.all(np.abs(direct - want) <= 1e-5 * want + 1e-6 * 256)
     assert np.all(np.abs(fft - want) <= 2e-6 * want + 1e-9 * 256)
 
