@@ -48,11 +48,19 @@ __device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float 
     // the clamps below only bound the loop and the indices.
     const int max_steps = 2 * M - 2;
     while ((i + 1 != M || j + 1 != M) && step < max_steps) {
-        float diag = kLdtwInf, rght = kLdtwInf, down = kLdtwInf;
+        // The three candidate costs are computed unconditionally and the
+        // disallowed ones replaced by INFTY with selects: no divergent
+        // branches (their exec-mask bookkeeping ran on the scalar unit).
+        // |i+1 - j| <= W and |j+1 - i| <= W, with the path inside the band
+        // (|i - j| <= W), reduce to i - j < W and j - i < W.
         const bool vi = i + 1 < M, vj = j + 1 < M;
-        if (vi && vj) diag = lcost<L2>(qi1 - zj1);
-        if (vi && abs(i - j + 1) <= W) down = lcost<L2>(qi1 - zj);
-        if (vj && abs(j - i + 1) <= W) rght = lcost<L2>(qi - zj1);
+        const int32_t dij = i - j;
+        const float cd = lcost<L2>(qi1 - zj1);
+        const float cn = lcost<L2>(qi1 - zj);
+        const float cr = lcost<L2>(qi - zj1);
+        const float diag = vi && vj ? cd : kLdtwInf;
+        const float down = vi && dij < W ? cn : kLdtwInf;
+        const float rght = vj && -dij < W ? cr : kLdtwInf;
         const bool dsd = diag < down;
         const bool dsr = diag < rght;
         const bool dnr = down < rght;
