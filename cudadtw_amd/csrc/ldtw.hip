@@ -31,7 +31,7 @@ __device__ __forceinline__ float lcost(float t) {
         return t < 0 ? -t : t;
 }
 
-template <bool L2>
+template <bool L2, bool QL>
 __device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
                              const float* __restrict__ q, int32_t M, int32_t W, int mode,
                              const Ctrl* ctrl, bool& alive) {
@@ -78,7 +78,16 @@ __device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float 
             adv_i = false;
             adv_j = true;
         }
-        if (adv_i && i + 1 < M) {
+        if constexpr (QL) {
+            // query in LDS: read the next value every step and select, no
+            // divergent branch
+            const bool ai = adv_i && i + 1 < M;
+            const float qn = q[i + 2 < M ? i + 2 : M - 1];
+            const float qn1 = i + 2 < M ? qn : 0.0f;
+            qi = ai ? qi1 : qi;
+            qi1 = ai ? qn1 : qi1;
+            i = ai ? i + 1 : i;
+        } else if (adv_i && i + 1 < M) {
             ++i;
             qi = qi1;
             qi1 = i + 1 < M ? q[i + 1] : 0.0f;
@@ -133,7 +142,7 @@ __global__ __launch_bounds__(64) void ldtw_kernel(
         const int64_t gc = valid ? g : count - 1;
         const int64_t w = idx ? static_cast<int64_t>(idx[gc]) : a.base + gc * a.stride;
         bool alive = valid;
-        const float d = ldtw_window<L2>(s + w, avg[w], sd[w], q, a.m, W, a.mode, ctrl, alive);
+        const float d = ldtw_window<L2, QL>(s + w, avg[w], sd[w], q, a.m, W, a.mode, ctrl, alive);
         if (a.mode == kModeProfile || a.mode == kModeSeed) {
             if (valid) out[g] = d;
         }
