@@ -2,7 +2,9 @@
 // (GPU-ldtw.cu): greedy LDTW of the z-normalised query against every window.
 //   gpu_ldtw query.bin subject.bin M N P
 // Raw FP32 files; W = M * (P * 0.01) truncated (GPU-ldtw.cu:19).  Prints the
-// reference's determine_min line for the GPU result.
+// reference's determine_min line for the GPU result, then -- as GPU-ldtw.cu
+// does (:108-141) -- the same search by the host paths on all cores and on
+// one core over the GPU's window statistics, as a consistency check.
 #include "common.hpp"
 
 int main(int argc, char* argv[]) {
@@ -38,6 +40,21 @@ int main(int argc, char* argv[]) {
         std::cout << "Miliseconds to find best alignment on the GPU: " << ta.ms() << std::endl;
         std::vector<float> r(static_cast<size_t>(L));
         HIP_OK(hipMemcpy(r.data(), R, sizeof(float) * L, hipMemcpyDeviceToHost));
+        determine_min<true>(r.data(), L);
+
+        std::vector<float> avgs(static_cast<size_t>(L)), stds(static_cast<size_t>(L));
+        HIP_OK(hipMemcpy(avgs.data(), AvgS, sizeof(float) * L, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(stds.data(), StdS, sizeof(float) * L, hipMemcpyDeviceToHost));
+        std::fill(r.begin(), r.end(), 0.0f);
+        Timer to;
+        omp_z_stream_ldtw<true>(q.data(), M, s.data(), N, avgs.data(), stds.data(), r.data(), W);
+        std::cout << "Miliseconds to find best alignment on all cores: " << to.ms() << std::endl;
+        determine_min<true>(r.data(), L);
+        std::fill(r.begin(), r.end(), 0.0f);
+        Timer t1;
+        z_stream_ldtw<true>(q.data(), M, s.data(), N, avgs.data(), stds.data(), r.data(), W);
+        std::cout << "Miliseconds to find best alignment on a single core: " << t1.ms()
+                  << std::endl;
         determine_min<true>(r.data(), L);
         HIP_OK(hipFree(S));
         HIP_OK(hipFree(AvgS));

@@ -354,14 +354,15 @@ int check_common(mdtw_batch* b, const float* s, int64_t n, int32_t metric, int32
     if (metric != MDTW_METRIC_L1 && metric != MDTW_METRIC_L2) return MDTW_ERR_INVALID_ARGUMENT;
     for (int32_t i = 0; i < b->nq; ++i) {
         if (n < b->m[i]) return MDTW_ERR_STREAM_TOO_SHORT;
-        if (n - b->m[i] + 1 > 0xffffffffll) return MDTW_ERR_UNSUPPORTED;
+        if (n - b->m[i] + 1 > kMaxSearchWindows) return MDTW_ERR_UNSUPPORTED;
     }
     return MDTW_OK;
 }
 
-std::vector<int64_t> excl_of(const mdtw_batch* b, const int64_t* h_excl) {
+std::vector<int64_t> excl_of(const mdtw_batch* b, const int64_t* h_excl, int64_t n) {
     std::vector<int64_t> ex(b->nq);
-    for (int32_t i = 0; i < b->nq; ++i) ex[i] = h_excl ? h_excl[i] : b->m[i] / 2;
+    for (int32_t i = 0; i < b->nq; ++i)
+        ex[i] = clamp_excl(h_excl ? h_excl[i] : b->m[i] / 2, n - b->m[i] + 1);
     return ex;
 }
 
@@ -479,7 +480,7 @@ int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n, int32_t met
     if (rc != MDTW_OK) return rc;
     DeviceGuard guard(b->e->device);
     if (cell_budget <= 0) cell_budget = int64_t(1) << 39;
-    const std::vector<int64_t> ex = excl_of(b, nullptr);
+    const std::vector<int64_t> ex = excl_of(b, nullptr, n);
     rc = ensure_ws(b, slot_bytes(b, n, k, ex));
     if (rc != MDTW_OK) return rc;
     // Queries per bucket; the median-cost one represents the bucket.
@@ -536,7 +537,7 @@ int mdtw_batch_search(mdtw_batch* b, const float* d_series, int64_t n, int32_t m
     if (!h_pos || !h_dist) return MDTW_ERR_INVALID_ARGUMENT;
     mdtw_engine* e = b->e;
     DeviceGuard guard(e->device);
-    const std::vector<int64_t> ex = excl_of(b, h_excl);
+    const std::vector<int64_t> ex = excl_of(b, h_excl, n);
     rc = ensure_ws(b, slot_bytes(b, n, k, ex));
     if (rc == MDTW_OK) rc = ensure_outputs(b, k);
     if (rc != MDTW_OK) return rc;

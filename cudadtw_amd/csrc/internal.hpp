@@ -26,6 +26,19 @@ constexpr int kWave = 64;
 constexpr int kRegBandMaxW = 57;
 constexpr int kMaxBandW = 1023;
 
+// Largest number of windows one search covers.  Candidate lists hold window
+// indices as int32 and the kernels widen them with a sign extension, so every
+// window index of a search must be below 2^31.
+constexpr int64_t kMaxSearchWindows = 0x7fffffffll;
+
+// Exclusion zone as a search uses it: negative means none (0); a zone of L or
+// more windows excludes every other window of an L-window search, exactly as
+// a zone of L does, so it is capped there.  With L <= kMaxSearchWindows the
+// bucket length 2 * excl stays below 2^32 (the device divides in 32 bits).
+inline int64_t clamp_excl(int64_t excl, int64_t L) {
+    return excl < 0 ? 0 : (excl > L ? L : excl);
+}
+
 // Control block in device memory, one per engine.
 struct Ctrl {
     unsigned long long best;     // (float bits << 32) | window index

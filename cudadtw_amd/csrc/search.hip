@@ -320,7 +320,8 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     const int32_t m = e->m;
     if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
     const int64_t L = n - m + 1;
-    if (L > 0xffffffffll) return MDTW_ERR_UNSUPPORTED;
+    if (L > kMaxSearchWindows) return MDTW_ERR_UNSUPPORTED;
+    excl = clamp_excl(excl, L);
     const int32_t wc = std::min(w, m - 1);
     if (algo == MDTW_ALGO_CDTW && wc > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
     for (int32_t i = 0; i < k; ++i) {

@@ -16,7 +16,8 @@
 //
 // MDTW_STATS_REFCOMPAT.  Restates the reference arithmetic: per 10^7-sample
 // epoch with M-1 overlap (stats.cuh:120-141), a global inclusive FP64 prefix
-// with P[0] = 0 (cub_util.cuh:29-54), (P[i+M] - P[i]) / M in FP64 stored as
+// with P[0] = 0 written over the epoch's first sample, which the in-place
+// scan then reads as 0 (cub_util.cuh:29-54), (P[i+M] - P[i]) / M in FP64 stored as
 // FP32 (stats.cuh:30-38), sigma = sqrt(fmaf(-X, X, X2)) or 1 (stats.cuh:40-52,
 // nvcc's default contraction of X2 - X*X).
 #include <algorithm>
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void ref_block_sums(
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         int64_t k = b0 + v;
-        if (k < len) {
+        if (k > 0 && k < len) {  // sample 0: see ref_prefix
             double d = static_cast<double>(x[k]);
             a1 += d;
             a2 += d * d;
@@ -370,7 +371,11 @@ __global__ __launch_bounds__(kThreads) void ref_prefix(const float* __restrict__
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
         int64_t k = b0 + v;
-        double d = k < len ? static_cast<double>(x[k]) : 0.0;
+        // The reference scans in place (prefix_sum(Aprefix, Aprefix, N),
+        // stats.cuh:100-101): the 0 it stores as P[0] (cub_util.cuh:35-36)
+        // overwrites the epoch's first sample before the scan reads it, so
+        // that sample counts as 0 -- only the epoch's first window changes.
+        double d = k > 0 && k < len ? static_cast<double>(x[k]) : 0.0;
         a1 += d;
         a2 += d * d;
         v1[v] = a1;

@@ -84,7 +84,8 @@ struct Cand {
 struct mdtw_stream {
     mdtw_engine* e = nullptr;
     int32_t m = 0, w = 0, algo = 0, metric = 0, stats_mode = 0, k = 1;
-    int64_t excl = 0;
+    int64_t excl = 0;        // exclusion zone of the whole stream (host selection)
+    int64_t excl_epoch = 0;  // the same zone as one epoch's search sees it (<= ew)
     int64_t ew = 0;  // windows per epoch
     int64_t es = 0;  // samples per epoch buffer
     uint64_t query_hash = 0;
@@ -200,9 +201,15 @@ int make_stream(mdtw_engine* e, int32_t w, int32_t algo, int32_t metric, int32_t
     const int32_t wc = std::min(w, m - 1);
     if (algo == MDTW_ALGO_CDTW && wc > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
     if (epoch_windows == 0) epoch_windows = kEpochDefault;
+    // An epoch is one search: its window indices must stay below 2^31.
     if (epoch_windows < kEpochAlign || epoch_windows % kEpochAlign != 0 ||
-        epoch_windows > (int64_t(1) << 31))
+        epoch_windows > kMaxSearchWindows)
         return MDTW_ERR_INVALID_ARGUMENT;
+    // Zones beyond 2^61 windows behave as 2^61 on any stream that can exist
+    // and keep 2 * excl inside int64; within one epoch a zone of ew or more
+    // excludes every other window, as a zone of ew does.
+    excl = std::min<int64_t>(excl, int64_t(1) << 61);
+    const int64_t excl_epoch = clamp_excl(excl, epoch_windows);
 
     DeviceGuard guard(e->device);
     mdtw_stream* s = new mdtw_stream();
@@ -214,6 +221,7 @@ int make_stream(mdtw_engine* e, int32_t w, int32_t algo, int32_t metric, int32_t
     s->stats_mode = stats_mode;
     s->k = k;
     s->excl = excl;
+    s->excl_epoch = excl_epoch;
     s->ew = epoch_windows;
     s->es = epoch_windows + m - 1;
     s->query_hash = query_hash_of(e);
@@ -221,7 +229,8 @@ int make_stream(mdtw_engine* e, int32_t w, int32_t algo, int32_t metric, int32_t
     for (int b = 0; b < 2 && rc == MDTW_OK; ++b)
         rc = dev_alloc(reinterpret_cast<void**>(&s->d_buf[b]), sizeof(float) * s->es);
     if (rc == MDTW_OK)
-        rc = dev_alloc(reinterpret_cast<void**>(&s->d_ws), search_ws_bytes(s->ew, k, excl, true));
+        rc = dev_alloc(reinterpret_cast<void**>(&s->d_ws),
+                       search_ws_bytes(s->ew, k, excl_epoch, true));
     if (rc == MDTW_OK) rc = dev_alloc(reinterpret_cast<void**>(&s->d_ctrl), sizeof(Ctrl));
     if (rc == MDTW_OK && hipHostMalloc(reinterpret_cast<void**>(&s->h_ctrl), sizeof(Ctrl),
                                        hipHostMallocDefault) != hipSuccess) {
@@ -352,7 +361,7 @@ int run_epoch(mdtw_stream* s, int64_t nsamples) {
     p.algo = s->algo;
     p.metric = s->metric;
     p.k = s->k;
-    p.excl = s->excl;
+    p.excl = s->excl_epoch;
     p.ctrl = s->d_ctrl;
     p.ws = s->d_ws;
     if (s->k == 1 && s->best_pos >= 0) {
@@ -540,12 +549,30 @@ int mdtw_stream_restore(mdtw_engine* e, const void* h_buf, size_t size, mdtw_str
     if (std::memcmp(h.magic, kMagic, sizeof(kMagic)) != 0) return MDTW_ERR_INVALID_ARGUMENT;
     if (!e->has_query) return MDTW_ERR_NO_QUERY;
     if (h.m != e->m || h.query_hash != query_hash_of(e)) return MDTW_ERR_INVALID_ARGUMENT;
+    // Every field is checked before it sizes anything: a truncated or corrupt
+    // buffer is an argument error, never an allocation or an overflow.
+    if (h.ew < kEpochAlign || h.ew % kEpochAlign != 0 || h.ew > kMaxSearchWindows)
+        return MDTW_ERR_INVALID_ARGUMENT;
     if (h.fill < 0 || h.ncand < 0 || h.fill > h.ew + h.m - 1 || h.base < 0 ||
         h.base % kEpochAlign != 0)
         return MDTW_ERR_INVALID_ARGUMENT;
-    const size_t need = sizeof(SavedHeader) + sizeof(float) * h.fill +
-                        (sizeof(float) + sizeof(int64_t)) * static_cast<size_t>(h.ncand);
-    if (size < need) return MDTW_ERR_INVALID_ARGUMENT;
+    const size_t fill_bytes = sizeof(float) * static_cast<size_t>(h.fill);
+    if (size - sizeof(SavedHeader) < fill_bytes) return MDTW_ERR_INVALID_ARGUMENT;
+    const size_t per_cand = sizeof(float) + sizeof(int64_t);
+    if (static_cast<uint64_t>(h.ncand) > (size - sizeof(SavedHeader) - fill_bytes) / per_cand)
+        return MDTW_ERR_INVALID_ARGUMENT;
+    // Windows searched so far start at 0 and end where the buffered samples
+    // begin; positions are windows already searched.
+    if (h.windows_done < 0 || h.windows_done > h.base) return MDTW_ERR_INVALID_ARGUMENT;
+    if (h.best_pos < -1 || h.best_pos >= h.base) return MDTW_ERR_INVALID_ARGUMENT;
+    {
+        const char* cp = static_cast<const char*>(h_buf) + sizeof(h) + fill_bytes;
+        for (int64_t i = 0; i < h.ncand; ++i, cp += per_cand) {
+            int64_t pos;
+            std::memcpy(&pos, cp + sizeof(float), sizeof(pos));
+            if (pos < 0 || pos >= h.base) return MDTW_ERR_INVALID_ARGUMENT;
+        }
+    }
     mdtw_stream* s = nullptr;
     int rc = make_stream(e, h.w, h.algo, h.metric, h.stats_mode, h.k, h.excl, h.ew, &s);
     if (rc != MDTW_OK) return rc;
@@ -571,7 +598,12 @@ int mdtw_stream_restore(mdtw_engine* e, const void* h_buf, size_t size, mdtw_str
     s->keogh = h.keogh;
     s->dp = h.dp;
     s->abandoned = h.abandoned;
-    s->cand.resize(static_cast<size_t>(h.ncand));
+    try {
+        s->cand.resize(static_cast<size_t>(h.ncand));
+    } catch (...) {
+        delete s;
+        return MDTW_ERR_ALLOC;
+    }
     for (Cand& c : s->cand) {
         std::memcpy(&c.d, in, sizeof(float));
         std::memcpy(&c.pos, in + sizeof(float), sizeof(int64_t));

@@ -33,8 +33,10 @@
 #ifndef MDTW_COMPAT_HPP
 #define MDTW_COMPAT_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -279,6 +281,161 @@ inline void tsh2_ldtw(const float* S, int M, int N, float* R, int W, void* strea
 }
 
 }  // namespace mdtw_compat
+
+// ---- the reference's CPU paths (host code, as in the reference) ----------------
+//
+// The reference ships CPU versions of its DP and greedy LDTW for its drivers'
+// and tests' comparisons (GPU-ldtw.cu:108-141 prints them beside the GPU
+// answer; tests/test_dtw.cu:129-130 compares against them).  They are host
+// loops here too, independent of the device path.  Rounding: every operation
+// is a separate IEEE single-precision step in the order of the reference's
+// expressions; built without FP contraction across statements (clang's
+// default), they give the device kernels' bits for LDTW and the unfused host
+// rounding of the reference's sequential_cdtw.
+
+namespace mdtw_compat {
+namespace host {
+
+inline float cost(float d, bool l2) {
+    if (l2) {
+        const float c = d * d;
+        return c;
+    }
+    return d < 0 ? -d : d;
+}
+
+// One window of the greedy walk (ldtw.cuh:18-127): from (0,0) to (M-1,M-1),
+// each step takes the smallest local cost among the diagonal, the cell below
+// (i+1, j) and the cell to the right (i, j+1), band-checked; the diagonal only
+// when strictly smaller than both, then below when not above the diagonal and
+// strictly below the right cell, else right.  zs(j) is the window's j-th
+// (z-normalised) sample.
+template <class Z>
+inline float greedy_walk(const float* q, Z zs, int M, int W, bool l2) {
+    const float kFar = 1e10f;  // INFTY, ldtw.cuh:8
+    int i = 0, j = 0;
+    float d = cost(q[0] - zs(0), l2);
+    while (i + 1 != M || j + 1 != M) {
+        float dg = kFar, dn = kFar, rt = kFar;
+        if (i + 1 < M && j + 1 < M) dg = cost(q[i + 1] - zs(j + 1), l2);
+        if (i + 1 < M && std::abs(i - j + 1) <= W) dn = cost(q[i + 1] - zs(j), l2);
+        if (j + 1 < M && std::abs(j - i + 1) <= W) rt = cost(q[i] - zs(j + 1), l2);
+        if (dg < dn && dg < rt) {
+            d += dg;
+            ++i;
+            ++j;
+        } else if (!(dg < dn) && dn < rt) {
+            d += dn;
+            ++i;
+        } else {
+            d += rt;
+            ++j;
+        }
+    }
+    return d;
+}
+
+template <bool Euclidean>
+inline void ldtw_windows(const float* query, int M, const float* subject, int N, const float* avgs,
+                         const float* stds, float* result, int W, bool parallel) {
+    const long L = static_cast<long>(N) - M + 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) if (parallel)
+#endif
+    for (long k = 0; k < L; ++k) {
+        const float* x = subject + k;
+        if (avgs) {
+            const float mu = avgs[k], sg = stds[k];
+            result[k] = greedy_walk(query, [&](int j) { return (x[j] - mu) / sg; }, M, W,
+                                    Euclidean);
+        } else {
+            result[k] = greedy_walk(query, [&](int j) { return x[j]; }, M, W, Euclidean);
+        }
+    }
+}
+
+// Banded DP of one window over two rolling rows (cdtw.cuh:23-88): row i of the
+// query against the band of the window, +inf outside it, D[0][0] = 0.
+template <bool euclidean>
+inline float cdtw_window(const float* zq, const float* x, float mu, float sg, int M, int W,
+                         std::vector<float>& rows) {
+    const int lane = M + 1;
+    rows.assign(2 * static_cast<size_t>(lane), INFINITY);
+    float* prev = rows.data();
+    float* cur = rows.data() + lane;
+    prev[0] = 0.0f;
+    for (int i = 1; i <= M; ++i) {
+        const int lo = std::max(0, i - W - 1), hi = std::min(lane, i + W + 1);
+        cur[lo] = INFINITY;
+        for (int j = lo + 1; j < hi; ++j) {
+            const float v = zq[i - 1] - (x[j - 1] - mu) / sg;
+            const float c = cost(v, euclidean);
+            const float up = prev[j] < cur[j - 1] ? prev[j] : cur[j - 1];
+            const float best = prev[j - 1] < up ? prev[j - 1] : up;
+            cur[j] = c + best;
+        }
+        if (hi < lane) cur[hi] = INFINITY;
+        std::swap(prev, cur);
+    }
+    return prev[M];
+}
+
+}  // namespace host
+}  // namespace mdtw_compat
+
+// cdtw.cuh:437-448 cpu_cdtw (sequential_cdtw :23-88 / openmp_cdtw :90-157):
+// host DP of the z-normalised query against the windows indices[0..indices_l)
+// of the host subject, squared for euclidean.
+template <bool euclidean, class value_t, class index_t, mdtw_compat::req_float<value_t> = 0>
+inline void cpu_cdtw(value_t* zquery, value_t* subject, value_t* avgS, value_t* stdS,
+                     value_t* cdtw, index_t M, index_t* indices, index_t indices_l, index_t W,
+                     bool openmp = true) {
+    const long n = static_cast<long>(indices_l);
+#ifdef _OPENMP
+#pragma omp parallel if (openmp)
+#endif
+    {
+        std::vector<float> rows;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (long p = 0; p < n; ++p) {
+            const long w = static_cast<long>(indices[p]);
+            cdtw[p] = mdtw_compat::host::cdtw_window<euclidean>(
+                zquery, subject + w, avgS[w], stdS[w], static_cast<int>(M), static_cast<int>(W),
+                rows);
+        }
+    }
+}
+
+// ldtw.cuh:151-174 z_stream_ldtw / omp_z_stream_ldtw: greedy LDTW of the
+// z-normalised query against every window of the host subject (window
+// statistics avgs / stds), on one core / on all cores.
+template <bool Euclidean, class value_t, class index_t, mdtw_compat::req_float<value_t> = 0>
+inline void z_stream_ldtw(const value_t* query, const index_t M, const value_t* subject,
+                          const index_t N, const value_t* avgs, const value_t* stds,
+                          value_t* result, const index_t W) {
+    mdtw_compat::host::ldtw_windows<Euclidean>(query, M, subject, N, avgs, stds, result, W, false);
+}
+template <bool Euclidean, class value_t, class index_t, mdtw_compat::req_float<value_t> = 0>
+inline void omp_z_stream_ldtw(const value_t* query, const index_t M, const value_t* subject,
+                              const index_t N, const value_t* avgs, const value_t* stds,
+                              value_t* result, const index_t W) {
+    mdtw_compat::host::ldtw_windows<Euclidean>(query, M, subject, N, avgs, stds, result, W, true);
+}
+// ldtw.cuh:130-149 stream_ldtw / omp_stream_ldtw: the same on raw values.
+template <bool Euclidean, class value_t, class index_t, mdtw_compat::req_float<value_t> = 0>
+inline void stream_ldtw(const value_t* query, const index_t M, const value_t* subject,
+                        const index_t N, value_t* result, const index_t W) {
+    mdtw_compat::host::ldtw_windows<Euclidean>(query, M, subject, N, nullptr, nullptr, result, W,
+                                               false);
+}
+template <bool Euclidean, class value_t, class index_t, mdtw_compat::req_float<value_t> = 0>
+inline void omp_stream_ldtw(const value_t* query, const index_t M, const value_t* subject,
+                            const index_t N, value_t* result, const index_t W) {
+    mdtw_compat::host::ldtw_windows<Euclidean>(query, M, subject, N, nullptr, nullptr, result, W,
+                                               true);
+}
 
 // ldtw.cuh:177-192 determine_min (host): strict '<', lowest index wins;
 // prints the reference's line.

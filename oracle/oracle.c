@@ -97,7 +97,13 @@ void or_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m) {
 /* ------------------------------------------------------------------------ */
 /* stats.cuh:85-113 for one epoch: FP64 inclusive prefix with P[0] = 0
  * (cub_util.cuh:29-54), FP64 window difference / M stored as FP32
- * (stats.cuh:30-38), FP32 finalize (stats.cuh:40-52). */
+ * (stats.cuh:30-38), FP32 finalize (stats.cuh:40-52).
+ * The reference scans in place: stats.cuh:100-101 call
+ * prefix_sum(Aprefix, Aprefix, N), whose first step writes 0 into Output[0]
+ * (cub_util.cuh:35-36), the same element as Input[0]; the scan that follows
+ * (:43-44) then reads that 0 instead of the epoch's first sample.  So
+ * P[k + 1] sums x[1..k], and only the epoch's first window, (P[M] - P[0]) / M,
+ * is affected: it leaves out x[0] (and x[0]^2 in the squares' prefix). */
 static void refcompat_epoch(const float* x, float* avg, float* sd, int32_t m,
                             int64_t len) {
     double* p1 = (double*)malloc(sizeof(double) * (size_t)(len + 1));
@@ -105,7 +111,7 @@ static void refcompat_epoch(const float* x, float* avg, float* sd, int32_t m,
     p1[0] = 0.0;
     p2[0] = 0.0;
     for (int64_t k = 0; k < len; ++k) {
-        double v = (double)x[k];
+        double v = k == 0 ? 0.0 : (double)x[k];
         p1[k + 1] = p1[k] + v;
         p2[k + 1] = p2[k] + v * v;
     }

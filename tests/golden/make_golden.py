@@ -9,6 +9,11 @@ container, where the reference tree is mounted:
 The binaries take FP64 files and ``data query m R n`` (R > 1 is an absolute
 band, UCR_DTW_BIN.cpp:419-423).  Only their printed (location, distance) is
 kept, with the SHA-256 of each input so a generator change is caught.
+
+Each case is run twice through UCR_DTW_BIN: at its own band W ("ucr_dtw") and
+at the band the reference's GPU-DTW driver derives from P = 10 percent,
+W = M*(P*0.01) truncated (GPU-dtw.cu:27) ("ucr_dtw_p10"), which is what a
+driver invoked as ``gpu_dtw q.bin s.bin M N 10`` searches with.
 """
 import hashlib
 import json
@@ -63,10 +68,14 @@ def main():
             s.astype(np.float64).tofile(dp)
             q.astype(np.float64).tofile(qp)
             dtw = run("UCR_DTW_BIN", [dp, qp, q.size, w, s.size])
+            w10 = int(q.size * (10 * 0.01))
+            dtw10 = run("UCR_DTW_BIN", [dp, qp, q.size, w10, s.size])
             ed = run("UCR_ED_BIN", [dp, qp, q.size, s.size])
             golden[name] = {"n": int(s.size), "m": int(q.size), "w": int(w),
                             "stream_sha256": sha(s), "query_sha256": sha(q),
                             "ucr_dtw": {"location": dtw[0], "distance": dtw[1]},
+                            "ucr_dtw_p10": {"w": w10, "location": dtw10[0],
+                                            "distance": dtw10[1]},
                             "ucr_ed": {"location": ed[0], "distance": ed[1]}}
             print(name, golden[name]["ucr_dtw"], golden[name]["ucr_ed"])
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "ucr_golden.json"),

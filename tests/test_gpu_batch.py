@@ -148,3 +148,47 @@ def test_batch_backward_equals_per_query(engine, raw):
         assert np.all(np.isinf(db[i][~keep]))
         np.testing.assert_array_equal(db[i][keep], r.distances[i][keep])
     b.close()
+
+
+# Mixed-batch shapes: lengths 64..1024, bands 5..20 % (three of them W >= 113,
+# the register-band kernel's wide range), half cut from the stream and
+# perturbed, half independent walks; top-8 with exclusion |Q|/2.
+C4_SPECS = [(64, 5, "cut"), (64, 20, "walk"), (96, 10, "cut"), (128, 15, "walk"),
+            (160, 20, "cut"), (200, 5, "walk"), (256, 10, "cut"), (300, 12, "walk"),
+            (384, 8, "cut"), (448, 6, "walk"), (512, 10, "cut"), (600, 20, "walk"),
+            (760, 15, "cut"), (1024, 15, "cut"), (900, 5, "walk"), (1000, 6, "cut")]
+
+
+def test_batch_mixed_shapes_against_oracle(engine):
+    """Every query's top-8 equals the greedy selection over the oracle's DP
+    (sequential_cdtw, cdtw.cuh:23-88, over the GPU's window statistics).  The
+    oracle evaluates every window whose LB_Keogh (bounds.cuh:111-144, a lower
+    bound of the same DP: test_oracle.py) does not exceed the batch's 8th
+    distance plus a rounding margin; every other window is farther than all
+    eight picks, so the greedy picks over the evaluated windows are the
+    picks over the whole profile (checked: all eight fall below the margin)."""
+    k = 8
+    s = synth.ecg_like(1 << 18, 44)
+    ds = dev(s)
+    qs, bands = make_queries(s, C4_SPECS, 50)
+    assert sum(int(w) >= 113 for w in bands) >= 3
+    b = engine.batch(qs, bands)
+    r = b.search(ds, k=k)
+    b.close()
+    for i, (q, w) in enumerate(zip(qs, bands)):
+        m, w = q.size, int(w)
+        avg, sd = engine.avg_std(ds, m)
+        torch.cuda.synchronize()
+        ha, hs = avg.cpu().numpy(), sd.cpu().numpy()
+        lo, up = O.envelope(q, w)
+        lb = O.lb_keogh(s, ha, hs, lo, up)
+        t = np.float64(r.distances[i, -1]) ** 2
+        limit = t * (1 + 1e-4) + 1e-3 if np.isfinite(t) else np.inf
+        cand = np.flatnonzero(lb <= limit).astype(np.int32)
+        prof = np.full(s.size - m + 1, np.inf, np.float32)
+        prof[cand] = O.cdtw(q, s, ha, hs, w, idx=cand)
+        idx, d2 = O.topk_greedy(prof, k, m // 2)
+        assert idx.size == k and np.all(d2 <= limit), f"query {i}"
+        np.testing.assert_array_equal(r.positions[i], idx, err_msg=f"query {i} (M={m}, W={w})")
+        np.testing.assert_array_equal(r.distances[i], np.sqrt(d2).astype(np.float32),
+                                      err_msg=f"query {i}")

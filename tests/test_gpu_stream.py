@@ -1,12 +1,13 @@
-"""Streaming search (mdtw_stream_*) against the one-shot search.
+"""Streaming search (mdtw_stream_*) against the oracle and the one-shot search.
 
 No reference API counterpart (the reference's epoch loops are internal:
 stats.cuh:115-142, UCR_DTW_BIN.cpp:579-707).  The bar: after finish, the
-streamed answer -- positions and distances -- is bit-identical to
-mdtw_search over the whole stream resident on the device (itself checked
-against the oracle's full profile in test_gpu_search.py), for every
-algorithm, k = 1 and top-k, any chunking of the pushes, matches that
-straddle epoch boundaries, and a save / restore in the middle.
+streamed answer -- positions and distances -- equals the greedy selection
+over the oracle's profile of the whole stream (cDTW and LDTW bit for bit,
+ED within 1e-5 of the FP64 oracle) across many epoch boundaries, and is
+bit-identical to mdtw_search over the whole stream resident on the device
+for every algorithm, k = 1 and top-k, any chunking of the pushes, matches
+that straddle epoch boundaries, and a save / restore in the middle.
 """
 import numpy as np
 import pytest
@@ -59,6 +60,67 @@ def test_stream_equals_one_shot(engine, algo, metric, k):
     assert got.info["windows"] == s.size - 128 + 1
     assert got.info["found"] == want.info["found"]
     st.close()
+
+
+def oracle_profile(engine, s, zq, w, algo, metric):
+    """The oracle's profile of every window: sequential_cdtw (cdtw.cuh:23-88)
+    or z_serial_ldtw (ldtw.cuh:74-127) over the GPU's window statistics (a
+    window's statistics depend only on its own samples and its 1024-window
+    tile, so every epoch sees the same values), or FP64 ED over FP64
+    two-pass statistics."""
+    m = zq.size
+    if algo == ALGO_ED:
+        _, _, a64, s64 = O.stats_fp64(s, m, want64=True)
+        return O.ed_f64(zq.astype(np.float64), s, a64, s64)
+    avg, sd = engine.avg_std(torch.from_numpy(s).cuda(), m)
+    torch.cuda.synchronize()
+    ha, hs = avg.cpu().numpy(), sd.cpu().numpy()
+    if algo == ALGO_CDTW:
+        return O.cdtw(zq, s, ha, hs, w, metric=metric, fused=1)
+    return O.ldtw(zq, s, ha, hs, w, metric=metric)
+
+
+@pytest.mark.parametrize("algo,metric,k,excl", [(ALGO_CDTW, METRIC_L2, 1, 0),
+                                                (ALGO_CDTW, METRIC_L2, 7, 48),
+                                                (ALGO_CDTW, METRIC_L1, 5, 2),
+                                                (ALGO_LDTW, METRIC_L2, 6, 64),
+                                                (ALGO_LDTW, METRIC_L1, 1, 0),
+                                                (ALGO_ED, METRIC_L2, 5, 64)])
+def test_stream_against_oracle_profile(engine, algo, metric, k, excl):
+    # Nine epochs of 8192 windows; pushes of random sizes cross every boundary.
+    m = 96
+    epoch = 8192
+    s = synth.ecg_like(9 * epoch + m - 1 - 333, 80 + k)
+    q, _ = synth.cut_query(s, m, 81)
+    zq = O.znormalize_f32(q)
+    w = 0 if algo == ALGO_ED else 9
+    engine.set_query(zq, w)
+    st = engine.stream(w, algo=algo, metric=metric, k=k, excl=excl, epoch_windows=epoch)
+    push_chunks(st, s, 90 + k)
+    st.finish()
+    got = st.result()
+    st.close()
+    assert got.info["windows"] == s.size - m + 1
+    prof = oracle_profile(engine, s, zq, w, algo, metric)
+    if algo == ALGO_ED:
+        # FP32 search against the FP64 oracle: the same picks, distances
+        # within 1e-5 relative (no near-ties at these ranks on this stream).
+        order = np.lexsort((np.arange(prof.size), prof))
+        idx, dist = [], []
+        for i in order:
+            if len(idx) == k:
+                break
+            if all(abs(int(i) - j) >= excl for j in idx):
+                idx.append(int(i))
+                dist.append(np.sqrt(prof[i]))
+        np.testing.assert_array_equal(got.positions, idx)
+        np.testing.assert_allclose(got.distances, dist, rtol=1e-5)
+        return
+    idx, d = O.topk_greedy(prof, k, excl)
+    np.testing.assert_array_equal(got.positions[:idx.size], idx)
+    want = np.sqrt(d).astype(np.float32) if metric == METRIC_L2 else d
+    np.testing.assert_array_equal(got.distances[:idx.size], want)
+    assert got.info["found"] == idx.size
 
 
 def test_matches_across_epoch_boundaries(engine):

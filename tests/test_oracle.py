@@ -126,30 +126,71 @@ def test_stats_fp64_constant_window_sigma_one():
     assert np.all(sd[:201] == 1.0) and np.all(avg[:201] == 5.5)
 
 
+def refcompat_epoch_starts(n, m, epoch=10 ** 7):
+    """First window of every epoch of stats.cuh:120-141's plan (epochs of
+    `epoch` samples advancing by epoch - M + 1; a short tail of at most M + 1
+    samples merges into the previous epoch)."""
+    width = epoch - m + 1
+    lowers = []
+    lower = 0
+    while lower < n:
+        upper = min(lower + epoch, n)
+        if upper - lower > m + 1 or not lowers:
+            lowers.append(lower)
+        lower += width
+    return lowers
+
+
+def refcompat_expected(s, m, epoch=10 ** 7):
+    """Exact statistics on exactly representable data, with the reference's
+    in-place scan at each epoch start (cub_util.cuh:35-36 writes the 0 of
+    P[0] over the epoch's first sample before the scan reads it, stats.cuh:
+    100-101): that epoch's first window sums x[lower + 1 .. lower + M) only.
+    The FP32 finalize is fmaf(-X, X, X2) as in stats.cuh:40-52."""
+    mu, sigma = np_window_stats(s, m)
+    x = mu.astype(np.float32)
+    x2 = (sigma.astype(np.float64) ** 2 + mu.astype(np.float64) ** 2).astype(np.float32)
+    for lo in refcompat_epoch_starts(s.size, m, epoch):
+        w = s[lo:lo + m].astype(np.float64).copy()
+        w[0] = 0.0
+        x[lo] = np.float32(w.sum() / m)
+        x2[lo] = np.float32((w * w).sum() / m)
+    # fmaf(-X, X, X2) with one rounding: on these small-integer windows X*X and
+    # X2 - X*X are exact in FP64, so one FP64 step then the FP32 cast is it.
+    v = (x2.astype(np.float64) - x.astype(np.float64) ** 2).astype(np.float32)
+    sd = np.where(v > 0, np.sqrt(v), np.float32(1.0)).astype(np.float32)
+    return x, sd
+
+
 def test_stats_refcompat_exact_on_small_integers():
     # Small integers: every prefix, window mean and FP32 finalize is exact, so
-    # the reference arithmetic must give the exact population statistics.
+    # the reference arithmetic must give the exact population statistics --
+    # except at the epoch start, where the in-place scan drops x[0].
     rng = np.random.default_rng(0)
     s = rng.integers(-8, 9, size=5000).astype(np.float32)
+    s[0] = 7.0  # a non-zero first sample, so its omission shows
     m = 16
     avg, sd = O.stats_refcompat(s, m)
-    mu, sigma = np_window_stats(s, m)
-    assert np.array_equal(avg, mu.astype(np.float32))
-    var = (sigma ** 2).astype(np.float32)
-    expect = np.where(var > 0, np.sqrt(var), 1.0).astype(np.float32)
-    assert np.max(np.abs(sd - expect) / expect) < 2e-7
+    ex, esd = refcompat_expected(s, m)
+    assert np.array_equal(avg, ex)
+    assert np.array_equal(sd, esd)
+    mu, _ = np_window_stats(s, m)
+    assert avg[0] != np.float32(mu[0]) and np.array_equal(avg[1:], mu[1:].astype(np.float32))
 
 
 def test_stats_refcompat_epochs_cover_every_window():
-    # Tiny epochs exercise the epoch/overlap/merge plan of stats.cuh:120-141;
-    # on exact data the result cannot depend on where epochs split.
+    # Tiny epochs exercise the epoch/overlap/merge plan of stats.cuh:120-141:
+    # every window is computed once, exactly, except each epoch's first.
     rng = np.random.default_rng(1)
     s = rng.integers(-4, 5, size=3001).astype(np.float32)
+    s[s == 0] = 3.0  # no zero samples: every dropped first sample shows
     m = 32
-    ref = O.stats_refcompat(s, m)
     for epoch in (34, 35, 100, 257, 3001, 10 ** 7):
         got = O.stats_refcompat(s, m, epoch)
-        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), epoch
+        ex, esd = refcompat_expected(s, m, epoch)
+        assert np.array_equal(got[0], ex), epoch
+        assert np.array_equal(got[1], esd), epoch
+    assert refcompat_epoch_starts(3001, 32, 100) == list(range(0, 3001 - 33, 69))
 
 
 def test_stats_refcompat_loses_precision_on_offset_data():
