@@ -724,351 +724,3 @@ int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
     free(ord);
     return got;
 }
-
-/* ------------------------------------------------------------------------ */
-/* UCR Suite port (UCR_DTW_BIN.cpp).  FP64 throughout, as the original. */
-#define UCR_INF 1e20
-#define UCR_DIST(x, y) (((x) - (y)) * ((x) - (y)))
-
-/* UCR_DTW_BIN.cpp:129-176 lower_upper_lemire (same algorithm as
- * bounds.cuh:13-64, on doubles, with the circular deque replaced by a flat
- * one). */
-static void ucr_lemire(const double* t, int len, int r, double* l, double* u) {
-    int* du = (int*)malloc(sizeof(int) * (size_t)(len + 1));
-    int* dl = (int*)malloc(sizeof(int) * (size_t)(len + 1));
-    int uh = 0, ut = 0, lh = 0, lt = 0;
-    du[ut++] = 0;
-    dl[lt++] = 0;
-    for (int i = 1; i < len; i++) {
-        if (i > r) {
-            u[i - r - 1] = t[du[uh]];
-            l[i - r - 1] = t[dl[lh]];
-        }
-        if (t[i] > t[i - 1]) {
-            ut--;
-            while (ut > uh && t[i] > t[du[ut - 1]]) ut--;
-        } else {
-            lt--;
-            while (lt > lh && t[i] < t[dl[lt - 1]]) lt--;
-        }
-        du[ut++] = i;
-        dl[lt++] = i;
-        if (i == 2 * r + 1 + du[uh])
-            uh++;
-        else if (i == 2 * r + 1 + dl[lh])
-            lh++;
-    }
-    for (int i = len; i < len + r + 1; i++) {
-        if (i - r - 1 >= 0) {
-            u[i - r - 1] = t[du[uh]];
-            l[i - r - 1] = t[dl[lh]];
-        }
-        if (i - du[uh] >= 2 * r + 1) uh++;
-        if (i - dl[lh] >= 2 * r + 1) lh++;
-    }
-    free(du);
-    free(dl);
-}
-
-/* UCR_DTW_BIN.cpp:183-224 */
-static double ucr_lb_kim_hierarchy(const double* t, const double* q, int j, int len,
-                                   double mean, double sdv, double bsf) {
-    double d, lb;
-    double x0 = (t[j] - mean) / sdv;
-    double y0 = (t[(len - 1 + j)] - mean) / sdv;
-    lb = UCR_DIST(x0, q[0]) + UCR_DIST(y0, q[len - 1]);
-    if (lb >= bsf) return lb;
-    double x1 = (t[(j + 1)] - mean) / sdv;
-    d = fmin(UCR_DIST(x1, q[0]), UCR_DIST(x0, q[1]));
-    d = fmin(d, UCR_DIST(x1, q[1]));
-    lb += d;
-    if (lb >= bsf) return lb;
-    double y1 = (t[(len - 2 + j)] - mean) / sdv;
-    d = fmin(UCR_DIST(y1, q[len - 1]), UCR_DIST(y0, q[len - 2]));
-    d = fmin(d, UCR_DIST(y1, q[len - 2]));
-    lb += d;
-    if (lb >= bsf) return lb;
-    double x2 = (t[(j + 2)] - mean) / sdv;
-    d = fmin(UCR_DIST(x0, q[2]), UCR_DIST(x1, q[2]));
-    d = fmin(d, UCR_DIST(x2, q[2]));
-    d = fmin(d, UCR_DIST(x2, q[1]));
-    d = fmin(d, UCR_DIST(x2, q[0]));
-    lb += d;
-    if (lb >= bsf) return lb;
-    double y2 = (t[(len - 3 + j)] - mean) / sdv;
-    d = fmin(UCR_DIST(y0, q[len - 3]), UCR_DIST(y1, q[len - 3]));
-    d = fmin(d, UCR_DIST(y2, q[len - 3]));
-    d = fmin(d, UCR_DIST(y2, q[len - 2]));
-    d = fmin(d, UCR_DIST(y2, q[len - 1]));
-    lb += d;
-    return lb;
-}
-
-/* UCR_DTW_BIN.cpp:235-252 */
-static double ucr_lb_keogh_cumulative(const int* order, const double* t,
-                                      const double* uo, const double* lo, double* cb,
-                                      int j, int len, double mean, double sdv,
-                                      double bsf) {
-    double lb = 0, x, d;
-    for (int i = 0; i < len && lb < bsf; i++) {
-        x = (t[(order[i] + j)] - mean) / sdv;
-        d = 0;
-        if (x > uo[i])
-            d = UCR_DIST(x, uo[i]);
-        else if (x < lo[i])
-            d = UCR_DIST(x, lo[i]);
-        lb += d;
-        cb[order[i]] = d;
-    }
-    return lb;
-}
-
-/* UCR_DTW_BIN.cpp:262-282 */
-static double ucr_lb_keogh_data_cumulative(const int* order, const double* qo,
-                                           double* cb, const double* l, const double* u,
-                                           int len, double mean, double sdv,
-                                           double bsf) {
-    double lb = 0, uu, ll, d;
-    for (int i = 0; i < len && lb < bsf; i++) {
-        uu = (u[order[i]] - mean) / sdv;
-        ll = (l[order[i]] - mean) / sdv;
-        d = 0;
-        if (qo[i] > uu)
-            d = UCR_DIST(qo[i], uu);
-        else if (qo[i] < ll)
-            d = UCR_DIST(qo[i], ll);
-        lb += d;
-        cb[order[i]] = d;
-    }
-    return lb;
-}
-
-/* UCR_DTW_BIN.cpp:288-354, with the two O(r) rows preallocated. */
-static double ucr_dtw(const double* A, const double* B, const double* cb, int m,
-                      int r, double bsf, double* cost, double* cost_prev) {
-    int i, j, k = 0;
-    double x, y, z, min_cost;
-    for (k = 0; k < 2 * r + 1; k++) cost[k] = UCR_INF;
-    for (k = 0; k < 2 * r + 1; k++) cost_prev[k] = UCR_INF;
-    for (i = 0; i < m; i++) {
-        k = imax(0, r - i);
-        min_cost = UCR_INF;
-        for (j = imax(0, i - r); j <= imin(m - 1, i + r); j++, k++) {
-            if ((i == 0) && (j == 0)) {
-                cost[k] = UCR_DIST(A[0], B[0]);
-                min_cost = cost[k];
-                continue;
-            }
-            if ((j - 1 < 0) || (k - 1 < 0))
-                y = UCR_INF;
-            else
-                y = cost[k - 1];
-            if ((i - 1 < 0) || (k + 1 > 2 * r))
-                x = UCR_INF;
-            else
-                x = cost_prev[k + 1];
-            if ((i - 1 < 0) || (j - 1 < 0))
-                z = UCR_INF;
-            else
-                z = cost_prev[k];
-            cost[k] = fmin(fmin(x, y), z) + UCR_DIST(A[i], B[j]);
-            if (cost[k] < min_cost) min_cost = cost[k];
-        }
-        if (i + r < m - 1 && min_cost + cb[i + r + 1] >= bsf)
-            return min_cost + cb[i + r + 1];
-        double* tmp = cost;
-        cost = cost_prev;
-        cost_prev = tmp;
-    }
-    k--;
-    return cost_prev[k];
-}
-
-typedef struct {
-    double v;
-    int i;
-} ucr_pair;
-
-static int ucr_cmp_abs_desc(const void* a, const void* b) {
-    const ucr_pair* x = (const ucr_pair*)a;
-    const ucr_pair* y = (const ucr_pair*)b;
-    double ax = fabs(x->v), ay = fabs(y->v);
-    if (ay > ax) return 1;
-    if (ay < ax) return -1;
-    return x->i - y->i;
-}
-
-/* UCR_DTW_BIN.cpp:527-707 main search loop. */
-int64_t or_ucr_dtw(const double* data, int64_t n, const double* query,
-                   int32_t m, int32_t r, double* dist, int64_t* counts) {
-    const int EPOCH = 100000;
-    double *q = malloc(sizeof(double) * m), *qo = malloc(sizeof(double) * m);
-    double *uo = malloc(sizeof(double) * m), *lo = malloc(sizeof(double) * m);
-    double *u = malloc(sizeof(double) * m), *l = malloc(sizeof(double) * m);
-    double *cb = calloc(m, sizeof(double)), *cb1 = calloc(m, sizeof(double));
-    double* cb2 = calloc(m, sizeof(double));
-    double *t = malloc(sizeof(double) * m * 2), *tz = malloc(sizeof(double) * m);
-    double* buffer = malloc(sizeof(double) * EPOCH);
-    double *u_buff = malloc(sizeof(double) * EPOCH), *l_buff = malloc(sizeof(double) * EPOCH);
-    double *cost = malloc(sizeof(double) * (2 * r + 1)), *cost_prev = malloc(sizeof(double) * (2 * r + 1));
-    int* order = malloc(sizeof(int) * m);
-    ucr_pair* qt = malloc(sizeof(ucr_pair) * m);
-
-    double ex = 0, ex2 = 0, mean, sdv, bsf = UCR_INF;
-    for (int i = 0; i < m; i++) {
-        ex += query[i];
-        ex2 += query[i] * query[i];
-        q[i] = query[i];
-    }
-    mean = ex / m;
-    sdv = ex2 / m;
-    sdv = sqrt(sdv - mean * mean);
-    for (int i = 0; i < m; i++) q[i] = (q[i] - mean) / sdv;
-    ucr_lemire(q, m, r, l, u);
-    for (int i = 0; i < m; i++) {
-        qt[i].v = q[i];
-        qt[i].i = i;
-    }
-    qsort(qt, (size_t)m, sizeof(ucr_pair), ucr_cmp_abs_desc);
-    for (int i = 0; i < m; i++) {
-        int o = qt[i].i;
-        order[i] = o;
-        qo[i] = q[o];
-        uo[i] = u[o];
-        lo[i] = l[o];
-    }
-
-    int64_t pos = 0, loc = 0;
-    int64_t kim = 0, keogh = 0, keogh2 = 0, it = 0, scanned = 0;
-    int done = 0, ep = 0;
-    while (!done) {
-        if (it == 0) {
-            for (int k = 0; k < m - 1; k++)
-                if (pos < n) buffer[k] = data[pos++];
-        } else {
-            for (int k = 0; k < m - 1; k++) buffer[k] = buffer[EPOCH - m + 1 + k];
-        }
-        ep = m - 1;
-        while (ep < EPOCH) {
-            if (pos >= n) break;
-            buffer[ep] = data[pos++];
-            ep++;
-        }
-        if (ep <= m - 1) {
-            done = 1;
-        } else {
-            ucr_lemire(buffer, ep, r, l_buff, u_buff);
-            ex = 0;
-            ex2 = 0;
-            for (int i = 0; i < ep; i++) {
-                double d = buffer[i];
-                ex += d;
-                ex2 += d * d;
-                t[i % m] = d;
-                t[(i % m) + m] = d;
-                if (i >= m - 1) {
-                    mean = ex / m;
-                    sdv = ex2 / m;
-                    sdv = sqrt(sdv - mean * mean);
-                    int j = (i + 1) % m;
-                    int I = i - (m - 1);
-                    double lb_kim = ucr_lb_kim_hierarchy(t, q, j, m, mean, sdv, bsf);
-                    if (lb_kim < bsf) {
-                        double lb_k = ucr_lb_keogh_cumulative(order, t, uo, lo, cb1, j, m,
-                                                              mean, sdv, bsf);
-                        if (lb_k < bsf) {
-                            for (int k = 0; k < m; k++) tz[k] = (t[(k + j)] - mean) / sdv;
-                            double lb_k2 = ucr_lb_keogh_data_cumulative(
-                                order, qo, cb2, l_buff + I, u_buff + I, m, mean, sdv, bsf);
-                            if (lb_k2 < bsf) {
-                                const double* src = lb_k > lb_k2 ? cb1 : cb2;
-                                cb[m - 1] = src[m - 1];
-                                for (int k = m - 2; k >= 0; k--) cb[k] = cb[k + 1] + src[k];
-                                double dd = ucr_dtw(tz, q, cb, m, r, bsf, cost, cost_prev);
-                                if (dd < bsf) {
-                                    bsf = dd;
-                                    loc = it * (EPOCH - m + 1) + i - m + 1;
-                                }
-                            } else
-                                keogh2++;
-                        } else
-                            keogh++;
-                    } else
-                        kim++;
-                    ex -= t[j];
-                    ex2 -= t[j] * t[j];
-                }
-            }
-            if (ep < EPOCH)
-                done = 1;
-            else
-                it++;
-        }
-    }
-    scanned = it * (EPOCH - m + 1) + ep;
-    if (dist) *dist = bsf;
-    if (counts) {
-        counts[0] = kim;
-        counts[1] = keogh;
-        counts[2] = keogh2;
-        counts[3] = scanned;
-    }
-    free(q); free(qo); free(uo); free(lo); free(u); free(l);
-    free(cb); free(cb1); free(cb2); free(t); free(tz); free(buffer);
-    free(u_buff); free(l_buff); free(cost); free(cost_prev); free(order); free(qt);
-    return loc;
-}
-
-/* UCR_ED_BIN.cpp:55-65 distance, :159-213 search loop. */
-int64_t or_ucr_ed(const double* data, int64_t n, const double* query,
-                  int32_t m, double* dist) {
-    double *Q = malloc(sizeof(double) * m), *T = malloc(sizeof(double) * 2 * m);
-    int* order = malloc(sizeof(int) * m);
-    ucr_pair* qt = malloc(sizeof(ucr_pair) * m);
-    double ex = 0, ex2 = 0, mean, sdv, bsf = UCR_INF;
-    for (int i = 0; i < m; ++i) {
-        ex += query[i];
-        ex2 += query[i] * query[i];
-    }
-    mean = ex / m;
-    sdv = ex2 / m;
-    sdv = sqrt(sdv - mean * mean);
-    for (int i = 0; i < m; i++) {
-        qt[i].v = (query[i] - mean) / sdv;
-        qt[i].i = i;
-    }
-    qsort(qt, (size_t)m, sizeof(ucr_pair), ucr_cmp_abs_desc);
-    for (int i = 0; i < m; i++) {
-        Q[i] = qt[i].v;
-        order[i] = qt[i].i;
-    }
-    int64_t loc = 0;
-    ex = ex2 = 0;
-    for (int64_t i = 0; i < n; ++i) {
-        double d = data[i];
-        ex += d;
-        ex2 += d * d;
-        T[i % m] = d;
-        T[(i % m) + m] = d;
-        if (i >= m - 1) {
-            int j = (int)((i + 1) % m);
-            mean = ex / m;
-            sdv = ex2 / m;
-            sdv = sqrt(sdv - mean * mean);
-            double sum = 0;
-            for (int k = 0; k < m && sum < bsf; k++) {
-                double x = (T[(order[k] + j)] - mean) / sdv;
-                sum += (x - Q[k]) * (x - Q[k]);
-            }
-            if (sum < bsf) {
-                bsf = sum;
-                loc = i - m + 1;
-            }
-            ex -= T[j];
-            ex2 -= T[j] * T[j];
-        }
-    }
-    if (dist) *dist = bsf;
-    free(Q); free(T); free(order); free(qt);
-    return loc;
-}

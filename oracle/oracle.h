@@ -35,7 +35,7 @@
  *                           or_znormalize_f64 for the raw-query form); W = 0
  *                           closed form 2(q - z); path cost equals or_cdtw_f64.
  *   or_ucr_dtw / or_ucr_ed  UCR_DTW_BIN / UCR_ED_BIN compiled from the
- *                           reference sources (oracle/_ref), location golden.
+ *   (cascade.c)             reference sources (oracle/_ref), location golden.
  */
 #ifndef MDTW_ORACLE_H
 #define MDTW_ORACLE_H
@@ -146,14 +146,17 @@ int64_t or_argmin(const float* x, int64_t n);
 int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
                        int64_t* out_idx, float* out_dist);
 
-/* Port of UCR_DTW_BIN.cpp:129-354,527-707 (LB_Kim hierarchy, LB_Keogh EQ and
- * EC, early-abandoning banded DTW, EPOCH streaming).  Returns the location;
- * *dist gets the squared distance; counts (optional) = {kim, keogh, keogh2,
- * dtw} pruning counters as UCR prints them. */
+/* cascade.c: the UCR-suite search restated from its paper (KDD 2012), pinned
+ * to UCR_DTW_BIN / UCR_ED_BIN compiled from the reference sources.  Best
+ * window of query (z-normalised here) in data with band r; *dist gets the
+ * squared distance; counts (optional) = {LB_Kim, LB_Keogh EQ, LB_Keogh EC
+ * prunes, windows}. */
 int64_t or_ucr_dtw(const double* data, int64_t n, const double* query,
                    int32_t m, int32_t r, double* dist, int64_t* counts);
-
-/* Port of UCR_ED_BIN.cpp:55-65,188-213 (early-abandoning z-normalised ED). */
+/* The same on `threads` host threads sharing one best-so-far. */
+int64_t or_ucr_dtw_par(const double* data, int64_t n, const double* query, int32_t m,
+                       int32_t r, int32_t threads, double* dist, int64_t* counts);
+/* Early-abandoning z-normalised ED with the query reordered by |q|. */
 int64_t or_ucr_ed(const double* data, int64_t n, const double* query,
                   int32_t m, double* dist);
 

@@ -58,6 +58,8 @@ def lib() -> C.CDLL:
             "or_argmin": (C.c_int64, [_F, C.c_int64]),
             "or_topk_greedy": (C.c_int32, [_F, C.c_int64, C.c_int32, C.c_int64, _L, _F]),
             "or_ucr_dtw": (C.c_int64, [_D, C.c_int64, _D, C.c_int32, C.c_int32, _D, _L]),
+            "or_ucr_dtw_par": (C.c_int64, [_D, C.c_int64, _D, C.c_int32, C.c_int32, C.c_int32,
+                                           _D, _L]),
             "or_ucr_ed": (C.c_int64, [_D, C.c_int64, _D, C.c_int32, _D]),
         }
         for name, (res, args) in sig.items():
@@ -235,6 +237,15 @@ def ucr_dtw(data, query, r):
     counts = np.zeros(4, np.int64)
     loc = lib().or_ucr_dtw(_d(data), data.size, _d(query), query.size, int(r), C.byref(dist),
                            counts.ctypes.data_as(_L))
+    return int(loc), float(dist.value), counts
+
+
+def ucr_dtw_par(data, query, r, threads):
+    data, query = _c64(data), _c64(query)
+    dist = C.c_double()
+    counts = np.zeros(4, np.int64)
+    loc = lib().or_ucr_dtw_par(_d(data), data.size, _d(query), query.size, int(r), int(threads),
+                               C.byref(dist), counts.ctypes.data_as(_L))
     return int(loc), float(dist.value), counts
 
 
