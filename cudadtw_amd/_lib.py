@@ -86,6 +86,8 @@ SIGNATURES = {
                                      _P]),
     "mdtw_search": (C.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64,
                               C.POINTER(_I64), _FP, C.POINTER(SearchInfo)]),
+    "mdtw_search_noalloc": (C.c_int, [_P, _P, _I64, _P, _P, _I32, _I32, _I32, _I32, _I32, _I64,
+                                      C.POINTER(_I64), _FP, C.POINTER(SearchInfo)]),
     "mdtw_batch_create": (C.c_int, [_P, _FP, C.POINTER(_I32), C.POINTER(_I32), _I32, _I32,
                                     C.POINTER(_P)]),
     "mdtw_batch_destroy": (C.c_int, [_P]),

@@ -48,13 +48,14 @@ int upload(mdtw_engine* e, void* dst, const void* src, size_t bytes) {
     return MDTW_OK;
 }
 
-// bounds.cuh:13-64 Lemire envelope (deques kept as flat index arrays; they
-// only grow at the back).  w >= m is clamped to m - 1 (global max/min).
-void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m) {
+// bounds.cuh:13-64 Lemire envelope (deques kept as flat index arrays of m + 1
+// entries; they only grow at the back).  w >= m is clamped to m - 1 (global
+// max/min).
+void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m, int* ub,
+                   int* lb) {
     if (m <= 0) return;
     if (w > m - 1) w = m - 1;
     if (w < 0) w = 0;
-    std::vector<int> ub(m + 1), lb(m + 1);
     int uh = 0, ut = 0, lh = 0, lt = 0;
     ub[ut++] = 0;
     lb[lt++] = 0;
@@ -85,9 +86,17 @@ void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m) {
     }
 }
 
+void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m) {
+    if (m <= 0) return;
+    std::vector<int> ub(static_cast<size_t>(m) + 1), lb(static_cast<size_t>(m) + 1);
+    host_envelope(q, w, lo, up, m, ub.data(), lb.data());
+}
+
 int ensure_envelope(mdtw_engine* e, int32_t w) {
     if (e->w_env == w) return MDTW_OK;
-    host_envelope(e->h_q.data(), w, e->h_lo.data(), e->h_up.data(), e->m);
+    int* dq = e->env_deques.data();
+    host_envelope(e->h_q.data(), w, e->h_lo.data(), e->h_up.data(), e->m, dq,
+                  dq + MDTW_MAX_QUERY + 1);
     int rc = upload(e, e->d_lo, e->h_lo.data(), sizeof(float) * e->m);
     if (rc == MDTW_OK) rc = upload(e, e->d_up, e->h_up.data(), sizeof(float) * e->m);
     if (rc == MDTW_OK) e->w_env = w;
@@ -182,6 +191,10 @@ int mdtw_create(int device, mdtw_engine** out) {
     e->h_q.resize(MDTW_MAX_QUERY);
     e->h_lo.resize(MDTW_MAX_QUERY);
     e->h_up.resize(MDTW_MAX_QUERY);
+    e->env_deques.resize(2 * (MDTW_MAX_QUERY + 1));
+    // The fixed scratch array of mdtw_search_noalloc, looked up on this device
+    // (current here) once; a failure is kept and reported by that entry point.
+    (void)scratch_setup(device);
     *out = e;
     return MDTW_OK;
 }

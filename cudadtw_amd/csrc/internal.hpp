@@ -75,6 +75,9 @@ struct mdtw_engine {
     float* d_lo = nullptr;
     float* d_up = nullptr;
     std::vector<float> h_q, h_lo, h_up;
+    // Index deques of the envelope (2 x (MDTW_MAX_QUERY + 1)), sized at
+    // creation so that building an envelope allocates nothing.
+    std::vector<int> env_deques;
     // Query padded with +inf for the register-band kernel, built per band.
     float* d_qx = nullptr;
     size_t qx_cap = 0;
@@ -115,6 +118,8 @@ struct DeviceGuard {
 int ws_reserve(mdtw_engine* e, size_t bytes, void** out);
 int upload(mdtw_engine* e, void* dst, const void* src, size_t bytes);
 void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m);
+// The same with caller-provided deques of m + 1 entries each (no allocation).
+void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m, int* ub, int* lb);
 int ensure_envelope(mdtw_engine* e, int32_t w);
 int ensure_qx(mdtw_engine* e, int32_t w);
 
@@ -230,6 +235,15 @@ struct SearchBufs {
     int64_t seeded = 0;
 };
 size_t search_ws_bytes(int64_t L, int32_t k, int64_t excl, bool own_stats);
+// mdtw_search's argument checks (status MDTW_OK or the error to return);
+// on success *excl is clamped to the stream and *wc is the band clamped to M - 1.
+int check_search_args(const mdtw_engine* e, const float* d_series, int64_t n, const float* d_avg,
+                      const float* d_std, int32_t w, int32_t algo, int32_t metric,
+                      int32_t stats_mode, int32_t k, const int64_t* h_pos, const float* h_dist,
+                      int64_t* excl, int32_t* wc);
+// Allocation-free search (noalloc.hip): the device's fixed scratch array is
+// looked up once per device at engine creation, with that device current.
+int scratch_setup(int device);
 // Host greedy top-k with exclusion over (distance, position) pairs sorted
 // ascending: a pair is taken unless |position - earlier pick| < excl.
 int32_t greedy_pick(const std::vector<std::pair<float, int64_t>>& sorted, int32_t k, int64_t excl,

@@ -296,10 +296,12 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
 
 }  // namespace mdtw
 
-extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, const float* d_avg,
-                           const float* d_std, int32_t w, int32_t algo, int32_t metric,
-                           int32_t stats_mode, int32_t k, int64_t excl, int64_t* h_pos,
-                           float* h_dist, mdtw_search_info* info) {
+namespace mdtw {
+
+int check_search_args(const mdtw_engine* e, const float* d_series, int64_t n, const float* d_avg,
+                      const float* d_std, int32_t w, int32_t algo, int32_t metric,
+                      int32_t stats_mode, int32_t k, const int64_t* h_pos, const float* h_dist,
+                      int64_t* excl, int32_t* wc_out) {
     if (!e || !d_series || !h_pos || !h_dist || w < 0) return w < 0 ? MDTW_ERR_BAND
                                                                     : MDTW_ERR_INVALID_ARGUMENT;
     if (!e->has_query) return MDTW_ERR_NO_QUERY;
@@ -321,9 +323,25 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
     const int64_t L = n - m + 1;
     if (L > kMaxSearchWindows) return MDTW_ERR_UNSUPPORTED;
-    excl = clamp_excl(excl, L);
     const int32_t wc = std::min(w, m - 1);
     if (algo == MDTW_ALGO_CDTW && wc > kMaxBandW) return MDTW_ERR_UNSUPPORTED;
+    *excl = clamp_excl(*excl, L);
+    *wc_out = wc;
+    return MDTW_OK;
+}
+
+}  // namespace mdtw
+
+extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, const float* d_avg,
+                           const float* d_std, int32_t w, int32_t algo, int32_t metric,
+                           int32_t stats_mode, int32_t k, int64_t excl, int64_t* h_pos,
+                           float* h_dist, mdtw_search_info* info) {
+    int32_t wc = 0;
+    int rc = check_search_args(e, d_series, n, d_avg, d_std, w, algo, metric, stats_mode, k, h_pos,
+                               h_dist, &excl, &wc);
+    if (rc != MDTW_OK) return rc;
+    const int32_t m = e->m;
+    const int64_t L = n - m + 1;
     for (int32_t i = 0; i < k; ++i) {
         h_pos[i] = -1;
         h_dist[i] = INFINITY;
@@ -333,7 +351,6 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
         return search_ed_fft(e, d_series, n, stats_mode, k, excl, h_pos, h_dist, info);
 
     // Query-side state first (host work and small uploads, outside timing).
-    int rc = MDTW_OK;
     if (algo != MDTW_ALGO_ED) {
         rc = ensure_envelope(e, wc);
         if (rc == MDTW_OK && algo == MDTW_ALGO_CDTW) rc = ensure_qx(e, wc);

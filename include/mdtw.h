@@ -203,6 +203,26 @@ int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n,
                 int64_t excl, int64_t* h_pos, float* h_dist,
                 mdtw_search_info* info);
 
+/* No reference counterpart; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * mdtw_search with the same arguments and results, except that it allocates
+ * nothing: its scratch (window statistics, candidate lists, kept keys, the
+ * padded query, the selected matches) lives in a fixed device array of
+ * 128 MiB that each GPU the library runs on holds until the process ends.
+ * Limits: algo CDTW, ED or LDTW (ED_FFT: MDTW_ERR_UNSUPPORTED); stats_mode
+ * FP64 or NONE, or statistics passed in (REFCOMPAT: MDTW_ERR_UNSUPPORTED);
+ * about 16 bytes per window (8 with d_avg/d_std given; 24 for top-k), so up
+ * to ~8.3M windows with statistics computed here, or ~5.5M for top-k --
+ * beyond: MDTW_ERR_WORKSPACE, before any launch; more than 8192 windows at or
+ * below the final top-k bound: MDTW_ERR_WORKSPACE (mdtw_search selects those
+ * on the host).  All calls on one GPU share its array: calls that overlap on
+ * one GPU (engines on different host threads) race. */
+int mdtw_search_noalloc(mdtw_engine* e, const float* d_series, int64_t n,
+                        const float* d_avg, const float* d_std, int32_t w,
+                        int32_t algo, int32_t metric, int32_t stats_mode, int32_t k,
+                        int64_t excl, int64_t* h_pos, float* h_dist,
+                        mdtw_search_info* info);
+
 /* ---- batched search ------------------------------------------------------- */
 /* No reference counterpart (the reference holds one query per process in
  * __constant__ memory, cdtw.cuh:9-11).  A batch is a set of queries of mixed
