@@ -34,6 +34,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--c4 sample|full|sho
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -55,33 +56,107 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
 def cpu_baseline(stream, zq, q_raw, m, w):
-    """Restated reference CPU path on a bounded sample (all host threads),
-    and the UCR cascade port (one thread)."""
+    """CPU comparators on this host, each on a bounded sample of the same
+    workload: the reference's CPU paths restated (brute-force banded DTW over
+    every window, cdtw.cuh:90-157 with OpenMP and :23-88 on one core; greedy
+    LDTW, ldtw.cuh:151-174, all cores and one core), the UCR-suite cascade
+    (project code from its paper: one thread, and one thread per core), and
+    UCR_DTW_BIN / UCR_ED_BIN compiled from the reference sources where they
+    are built (oracle/_ref).  Returns (cpu_baseline, extra comparators)."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    nwin = 1 << 19
-    sample = stream[:nwin + m - 1]
-    avg, sd = O.stats_fp64(sample, m)
+    threads = O.max_threads()
+    host = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_threads": threads}
+    c = cells(m, w)
+
+    def brute(nwin, nthreads):
+        O.set_threads(nthreads)
+        sample = stream[:nwin + m - 1]
+        avg, sd = O.stats_fp64(sample, m)
+        t0 = time.perf_counter()
+        O.cdtw(zq, sample, avg, sd, w, fused=0)
+        dt = time.perf_counter() - t0
+        O.set_threads(threads)
+        return nwin * c / dt / 1e9, dt
+
+    gcups, t_bf = brute(1 << 19, threads)
+    g1, t_1 = brute(1 << 15, 1)
+    cb = {"value": round(gcups, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
+          "sample": f"first {1 << 19} windows of the stream, M={m}, W={w}; brute-force banded "
+                    f"DTW over every window (cdtw.cuh:90-157 restated, OpenMP), {t_bf:.2f} s",
+          **host}
+    extra = {"host": host,
+             "cdtw_one_core": {"gcups": round(g1, 4), "cores": 1, "kind": "port",
+                               "sample": f"first {1 << 15} windows, sequential_cdtw "
+                                         f"(cdtw.cuh:23-88 restated), {t_1:.2f} s"}}
+    # greedy LDTW, all cores and one core (GPU-ldtw.cu:108-141)
+    for name, nwin, nthreads in (("ldtw_all_cores", 1 << 21, threads), ("ldtw_one_core", 1 << 17, 1)):
+        O.set_threads(nthreads)
+        sample = stream[:nwin + m - 1]
+        avg, sd = O.stats_fp64(sample, m)
+        t0 = time.perf_counter()
+        O.ldtw(zq, sample, avg, sd, w)
+        dt = time.perf_counter() - t0
+        extra[name] = {"windows_per_s": round(nwin / dt, 1),
+                       "g_steps_per_s": round(nwin * (2 * m - 2) / dt / 1e9, 4),
+                       "cores": nthreads, "kind": "port",
+                       "sample": f"first {nwin} windows, M={m}, W={w}, z_stream_ldtw "
+                                 f"(ldtw.cuh:151-174 restated), {dt:.2f} s"}
+    O.set_threads(threads)
+    # UCR-suite cascade: one thread (as UCR runs), one per core
+    q64 = q_raw.astype(np.float64)
+    for name, nu, nthreads in (("ucr_dtw_one_thread", 1 << 20, 1),
+                               ("ucr_dtw_all_threads", 1 << 22, threads)):
+        us = stream[:nu].astype(np.float64)
+        t0 = time.perf_counter()
+        if nthreads == 1:
+            loc, d2, _ = O.ucr_dtw(us, q64, w)
+        else:
+            loc, d2, _ = O.ucr_dtw_par(us, q64, w, nthreads)
+        dt = time.perf_counter() - t0
+        L_u = nu - m + 1
+        extra[name] = {"windows_per_s": round(L_u / dt, 1),
+                       "effective_gcups": round(L_u * c / dt / 1e9, 4), "cores": nthreads,
+                       "kind": "port", "sample": f"first {nu} samples, M={m}, W={w}, UCR cascade "
+                                                 f"(KDD 2012), {dt:.2f} s, location {loc}"}
+    ue = stream[:1 << 22].astype(np.float64)
     t0 = time.perf_counter()
-    O.cdtw(zq, sample, avg, sd, w, fused=0)
-    t_bf = time.perf_counter() - t0
-    gcups = nwin * cells(m, w) / t_bf / 1e9
-    nu = 1 << 21
-    us = stream[:nu].astype(np.float64)
-    t0 = time.perf_counter()
-    loc, d2, counts = O.ucr_dtw(us, q_raw.astype(np.float64), w)
-    t_ucr = time.perf_counter() - t0
-    L_ucr = nu - m + 1
-    return ({"value": round(gcups, 4), "unit": "GCUPS", "cores": threads, "kind": "port",
-             "sample": f"first {nwin} windows of the stream, M={m}, W={w}; brute-force "
-                       f"banded DTW over every window (cdtw.cuh:90-157 restated, OpenMP), "
-                       f"{t_bf:.2f} s"},
-            {"windows_per_s": round(L_ucr / t_ucr, 1),
-             "effective_gcups": round(L_ucr * cells(m, w) / t_ucr / 1e9, 4), "cores": 1,
-             "kind": "port", "sample": f"first {nu} samples, M={m}, W={w}, UCR_DTW cascade "
-                                       f"port, {t_ucr:.2f} s, location {loc}"})
+    loc, _ = O.ucr_ed(ue, q64)
+    dt = time.perf_counter() - t0
+    extra["ucr_ed_one_thread"] = {"windows_per_s": round((ue.size - m + 1) / dt, 1), "cores": 1,
+                                  "kind": "port",
+                                  "sample": f"first {ue.size} samples, M={m}, early-abandoning "
+                                            f"z-normalised ED, {dt:.2f} s, location {loc}"}
+    # the reference's own UCR programs, built from its sources
+    for name, binary, nu, r in (("ucr_dtw_bin", "UCR_DTW_BIN", 1 << 20, w),
+                                ("ucr_ed_bin", "UCR_ED_BIN", 1 << 22, None)):
+        us = stream[:nu].astype(np.float64)
+        t0 = time.perf_counter()
+        res = O.ucr_reference(binary, us, q64, r)
+        dt = time.perf_counter() - t0
+        if res is None:
+            extra[name] = {"kind": "reference", "skipped": "oracle/_ref not built on this host"}
+            continue
+        loc, dist, secs = res
+        secs = secs if secs else dt
+        extra[name] = {"windows_per_s": round((nu - m + 1) / secs, 1), "cores": 1,
+                       "kind": "reference",
+                       "sample": f"first {nu} samples, M={m}" + (f", R={r}" if r else "") +
+                                 f", {binary} from the reference sources, {secs:.2f} s "
+                                 f"(its own clock), location {loc}, distance {dist}"}
+    return cb, extra
 
 
 def timed(torch, fn, reps=1):
@@ -580,9 +655,9 @@ def main():
     out.update(extra)
     if not args.no_cpu:
         log("CPU baseline ...")
-        cb, ucr = cpu_baseline(stream, zq, q_raw, m, w)
+        cb, comparators = cpu_baseline(stream, zq, q_raw, m, w)
         out["cpu_baseline"] = cb
-        out["cpu_ucr_cascade"] = ucr
+        out["cpu_comparators"] = comparators
     eng.close()
     print(json.dumps(out), flush=True)
 

@@ -67,6 +67,12 @@ SIGNATURES = {
     "mdtw_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "mdtw_destroy": (C.c_int, [_P]),
     "mdtw_set_stream": (C.c_int, [_P, _P]),
+    "mdtw_set_workspace_limit": (C.c_int, [_P, C.c_size_t]),
+    "mdtw_avg_std_f64": (C.c_int, [_P, _P, _I64, _I32, _P, _P]),
+    "mdtw_ed_f64": (C.c_int, [_P, _P, _I64, C.POINTER(C.c_double), _I32, _P]),
+    "mdtw_ed_search_f64": (C.c_int, [_P, _P, _I64, C.POINTER(C.c_double), _I32,
+                                     C.POINTER(_I64), C.POINTER(C.c_double),
+                                     C.POINTER(SearchInfo)]),
     "mdtw_synchronize": (C.c_int, [_P]),
     "mdtw_status_string": (C.c_char_p, [C.c_int]),
     "mdtw_device_info": (C.c_int, [_P, C.c_char_p, _I32, C.POINTER(_I32)]),

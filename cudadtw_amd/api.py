@@ -35,16 +35,17 @@ def _fp(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
-def znormalize(series, double: bool = False) -> np.ndarray:
-    """stats.cuh:8-28 znormalize: returns the z-normalised FP32 copy,
+def znormalize(series, double: bool = False, keep_f64: bool = False) -> np.ndarray:
+    """stats.cuh:8-28 znormalize: returns the z-normalised copy,
     bit-identical to the reference's host routine.  double=False is
     value_t = float (the cDTW / LDTW drivers, GPU-dtw.cu:67); double=True is
-    value_t = double (the ED driver, GPU-ed.cu:62), rounded to FP32 after."""
+    value_t = double (the ED driver, GPU-ed.cu:62), rounded to FP32 after
+    unless keep_f64 (the FP64 ED entry points take it as is)."""
     if double:
         x = np.ascontiguousarray(series, dtype=np.float64).copy()
         check(_lib.load().mdtw_znormalize_f64(x.ctypes.data_as(C.POINTER(C.c_double)), x.size),
               "znormalize")
-        return x.astype(np.float32)
+        return x if keep_f64 else x.astype(np.float32)
     x = np.ascontiguousarray(series, dtype=np.float32).copy()
     check(_lib.load().mdtw_znormalize(_fp(x), x.size), "znormalize")
     return x
@@ -293,11 +294,67 @@ class Engine:
         r = self.search(subject, w, ALGO_CDTW, metric, 1, 0, avg, sd)
         return r.location, r.distance
 
-    def calculate_ed(self, subject, avg=None, sd=None):
+    def calculate_ed(self, subject, avg=None, sd=None, zquery64=None):
         """ed.cuh:71-153 calculate_ed: (location, distance) of the best
-        z-normalised Euclidean match."""
+        z-normalised Euclidean match.  A float64 ``subject`` takes the FP64
+        path of the reference's ED driver (GPU-ed.cu): ``zquery64`` is then
+        the FP64 z-normalised query (``znormalize(q, double=True,
+        keep_f64=True)``) and the window statistics are computed in FP64."""
+        if subject.dtype == self._torch.float64:
+            return self.ed_search_f64(subject, zquery64)
         r = self.search(subject, 0, ALGO_ED, METRIC_L2, 1, 0, avg, sd)
         return r.location, r.distance
+
+    # -- FP64 streams (the reference's ED driver) ----------------------------
+    def _f64(self, t, name):
+        torch = self._torch
+        if not (isinstance(t, torch.Tensor) and t.device == self._dev and t.dtype == torch.float64
+                and t.is_contiguous()):
+            raise TypeError(f"{name}: expected a contiguous float64 tensor on {self._dev}")
+        return C.c_void_p(t.data_ptr())
+
+    def avg_std_f64(self, series, m: int):
+        """stats.cuh:115-142 avg_std<double> on an FP64 stream: FP64 window
+        means and population sigmas."""
+        L = series.numel() - m + 1
+        avg = self._torch.empty(max(L, 1), dtype=self._torch.float64, device=self._dev)
+        sd = self._torch.empty(max(L, 1), dtype=self._torch.float64, device=self._dev)
+        self._stream()
+        check(self.lib.mdtw_avg_std_f64(self.h, self._f64(series, "series"), series.numel(),
+                                        int(m), self._f64(avg, "avg"), self._f64(sd, "std")),
+              "mdtw_avg_std_f64")
+        return avg[:L], sd[:L]
+
+    def ed_f64(self, series, zquery64):
+        """FP64 squared z-normalised ED of every window of an FP64 stream."""
+        q = np.ascontiguousarray(zquery64, dtype=np.float64)
+        L = series.numel() - q.size + 1
+        out = self._torch.empty(max(L, 1), dtype=self._torch.float64, device=self._dev)
+        self._stream()
+        check(self.lib.mdtw_ed_f64(self.h, self._f64(series, "series"), series.numel(),
+                                   q.ctypes.data_as(C.POINTER(C.c_double)), q.size,
+                                   self._f64(out, "out")), "mdtw_ed_f64")
+        return out[:L]
+
+    def ed_search_f64(self, series, zquery64):
+        """ed.cuh:71-153 calculate_ed on an FP64 stream: (location,
+        distance), FP64 throughout."""
+        q = np.ascontiguousarray(zquery64, dtype=np.float64)
+        pos = C.c_int64(-1)
+        dist = C.c_double()
+        info = _lib.SearchInfo()
+        self._stream()
+        check(self.lib.mdtw_ed_search_f64(self.h, self._f64(series, "series"), series.numel(),
+                                          q.ctypes.data_as(C.POINTER(C.c_double)), q.size,
+                                          C.byref(pos), C.byref(dist), C.byref(info)),
+              "mdtw_ed_search_f64")
+        return pos.value, dist.value
+
+    def set_workspace_limit(self, nbytes: int):
+        """Cap the engine's device workspace (0: none); calls needing more
+        raise MdtwError(ERR_WORKSPACE) before any launch."""
+        check(self.lib.mdtw_set_workspace_limit(self.h, int(nbytes)),
+              "mdtw_set_workspace_limit")
 
     # -- batched search ----------------------------------------------------
     def batch(self, zqueries, bands, pipelines: int = 0) -> "Batch":

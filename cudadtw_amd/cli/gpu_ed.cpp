@@ -1,9 +1,10 @@
 // gpu_ed -- command-line equivalent of the reference's GPU-ED driver
 // (GPU-ed.cu): best z-normalised Euclidean match.
 //   gpu_ed query.bin subject.bin M N
-// Raw FP64 files as the reference reads them (GPU-ed.cu:51-56); the query is
-// z-normalised in FP64 (znormalize<double>, GPU-ed.cu:62), then query and
-// stream are searched in FP32.  Prints the reference's result lines.
+// Raw FP64 files as the reference reads them (GPU-ed.cu:51-56), searched in
+// FP64 as the reference does: znormalize<double> (GPU-ed.cu:62), the query
+// set as FP64 (:72), avg_std<double> on the FP64 stream (:76), calculate_ed
+// (:82).  Prints the reference's result lines.
 #include "common.hpp"
 
 int main(int argc, char* argv[]) {
@@ -18,21 +19,24 @@ int main(int argc, char* argv[]) {
     try {
         std::cout << "\n= loading data =============================" << std::endl;
         Timer tl;
-        std::vector<double> zq64 = read_bin<double>(argv[1], M);
-        std::vector<double> s64 = read_bin<double>(argv[2], N);
-        znormalize(zq64.data(), M);
-        std::vector<float> zquery(zq64.begin(), zq64.end());
-        std::vector<float> subject(s64.begin(), s64.end());
+        std::vector<double> zquery = read_bin<double>(argv[1], M);
+        std::vector<double> subject = read_bin<double>(argv[2], N);
+        znormalize(zquery.data(), M);
         std::cout << "Miliseconds to load data: " << tl.ms() << std::endl;
 
         Timer ts;
-        float* Subject = nullptr;
-        HIP_OK(hipMalloc(&Subject, sizeof(float) * N));
-        HIP_OK(hipMemcpy(Subject, subject.data(), sizeof(float) * N, hipMemcpyHostToDevice));
-        mdtw_compat::set_query(zquery.data(), nullptr, nullptr, M, 0);
-        calculate_ed(Subject, static_cast<const float*>(nullptr), M, N);
+        double *Subject = nullptr, *AvgS = nullptr, *StdS = nullptr;
+        HIP_OK(hipMalloc(&Subject, sizeof(double) * N));
+        HIP_OK(hipMalloc(&AvgS, sizeof(double) * (N - M + 1)));
+        HIP_OK(hipMalloc(&StdS, sizeof(double) * (N - M + 1)));
+        HIP_OK(hipMemcpy(Subject, subject.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+        mdtw_compat::set_query(zquery.data(), M);
+        avg_std<double>(Subject, AvgS, StdS, M, N);
+        HIP_OK(hipFree(AvgS));
+        calculate_ed(Subject, StdS, M, N);
         std::cout << "Miliseconds to find best match: " << ts.ms() << std::endl;
         HIP_OK(hipFree(Subject));
+        HIP_OK(hipFree(StdS));
     } catch (const mdtw_compat::error& e) {
         std::cerr << e.what() << std::endl;
         return 1;

@@ -226,7 +226,12 @@ __global__ __launch_bounds__(64) void wide_kernel(
     while (g < count) {
         const int64_t w = idx ? static_cast<int64_t>(idx[g]) : a.base + g * a.stride;
         bool alive = true;
-        const float d = wide_dp<C, L2>(s + w, avg[w], sd[w], q, a.m, a.w, a.mode, ctrl, alive);
+        const float mu = avg[w];
+        float d = wide_dp<C, L2>(s + w, mu, sd[w], q, a.m, a.w, a.mode, ctrl, alive);
+        // A window holding a non-finite sample has NaN statistics and so a
+        // NaN distance; the band padding (fmaxf with -inf) would turn NaN
+        // cells into -inf, so the NaN is restored here.
+        if (!(mu == mu)) d = mu;
         if (lane == 0) {
             if (a.mode == kModeProfile || a.mode == kModeSeed) out[g] = d;
             if ((a.mode == kModeBest || a.mode == kModeSeed) && alive)

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -118,10 +119,11 @@ __device__ __forceinline__ void real_split(double2 zk, double2 zm, double2 wk, d
     *xm = cadd(cconj(e), cmul(wm, cconj(o)));
 }
 
-// The query's real spectrum Q[0..H] (q zero-padded to N = 2H).
-template <int LOGH>
+// The query's real spectrum Q[0..H] (q zero-padded to N = 2H); QT is the
+// query's type (FP32, or FP64 for the FP64-stream form).
+template <int LOGH, class QT>
 __global__ __launch_bounds__(kFftThreads) void query_spectrum_kernel(
-    const float* __restrict__ q, int32_t m, const double2* __restrict__ tw,
+    const QT* __restrict__ q, int32_t m, const double2* __restrict__ tw,
     double2* __restrict__ qs) {
     constexpr int H = 1 << LOGH;
     __shared__ double2 a[H];
@@ -156,20 +158,27 @@ struct FftScalars {
     double qsum;      // sum of q
 };
 
-template <int LOGH>
+// S: the stream's type, O: the profile's (FP32 for FP32 streams, FP64 for
+// FP64 ones).
+template <int LOGH, class S, class O>
 __global__ __launch_bounds__(kFftThreads) void ed_fft_kernel(
-    const float* __restrict__ s, const double* __restrict__ avg, const double* __restrict__ var,
-    const double2* __restrict__ tw, const double2* __restrict__ qs, float* __restrict__ out,
+    const S* __restrict__ s, const double* __restrict__ avg, const double* __restrict__ var,
+    const double2* __restrict__ tw, const double2* __restrict__ qs, O* __restrict__ out,
     FftScalars a) {
     constexpr int H = 1 << LOGH;
     constexpr int N = 2 * H;
     __shared__ double2 z[H];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * a.V;
-    const double c = static_cast<double>(s[base]);
+    // Non-finite samples enter the transform as 0 (their windows' statistics
+    // are NaN, so those windows' distances are NaN); one would otherwise
+    // spread through the whole block's transform.
+    const double c = isfinite(s[base]) ? static_cast<double>(s[base]) : 0.0;
     for (int n = threadIdx.x; n < H; n += kFftThreads) {
         const int64_t i0 = base + 2 * n;
-        const double v0 = i0 < a.n ? static_cast<double>(s[i0]) - c : 0.0;
-        const double v1 = i0 + 1 < a.n ? static_cast<double>(s[i0 + 1]) - c : 0.0;
+        const S x0 = i0 < a.n ? s[i0] : S(0);
+        const S x1 = i0 + 1 < a.n ? s[i0 + 1] : S(0);
+        const double v0 = isfinite(x0) && i0 < a.n ? static_cast<double>(x0) - c : 0.0;
+        const double v1 = isfinite(x1) && i0 + 1 < a.n ? static_cast<double>(x1) - c : 0.0;
         z[bitrev<LOGH>(n)] = {v0, v1};
     }
     __syncthreads();
@@ -212,7 +221,8 @@ __global__ __launch_bounds__(kFftThreads) void ed_fft_kernel(
         const double sig = va > 0.0 ? sqrt(va) : 1.0;
         const double sz2 = va > 0.0 ? m : 0.0;
         const double ed2 = a.q2 + sz2 - 2.0 * (dot - (mu - c) * a.qsum) / sig;
-        out[p] = static_cast<float>(ed2 > 0.0 ? ed2 : 0.0);
+        // below 0 (rounding) -> +0; NaN (a non-finite sample) stays NaN
+        out[p] = static_cast<O>(ed2 > 0.0 ? ed2 : (ed2 == ed2 ? 0.0 : ed2));
     }
 }
 
@@ -274,18 +284,48 @@ int fft_log(int32_t m) {
     return lg;
 }
 
-template <int LOGH>
-int launch_fft_pair(hipStream_t st, const float* s, const double* avg, const double* var,
-                    const double2* tw, const float* q, double2* qs, float* out,
-                    const FftScalars& a) {
-    hipLaunchKernelGGL(query_spectrum_kernel<LOGH>, dim3(1), dim3(kFftThreads), 0, st, q, a.m, tw,
-                       qs);
+template <int LOGH, class S, class O>
+int launch_fft_pair(hipStream_t st, const S* s, const double* avg, const double* var,
+                    const double2* tw, const S* q, double2* qs, O* out, const FftScalars& a) {
+    hipLaunchKernelGGL((query_spectrum_kernel<LOGH, S>), dim3(1), dim3(kFftThreads), 0, st, q,
+                       a.m, tw, qs);
     int rc = hip_ok(hipGetLastError());
     if (rc != MDTW_OK) return rc;
     const int64_t nb = (a.L + a.V - 1) / a.V;
-    hipLaunchKernelGGL(ed_fft_kernel<LOGH>, dim3(static_cast<unsigned>(nb)), dim3(kFftThreads), 0,
-                       st, s, avg, var, tw, qs, out, a);
+    hipLaunchKernelGGL((ed_fft_kernel<LOGH, S, O>), dim3(static_cast<unsigned>(nb)),
+                       dim3(kFftThreads), 0, st, s, avg, var, tw, qs, out, a);
     return hip_ok(hipGetLastError());
+}
+
+// Statistics, twiddles, query spectrum and the ED^2 profile of all L windows
+// (avg, var: L FP64 each; tw: N/2; qs: N/2 + 1 entries, N = 2^fft_log(m)).
+template <class S, class O>
+int run_ed_fft(hipStream_t st, const S* s, int64_t n, const S* q, int32_t m, double q2,
+               double qsum, double* avg, double* var, double2* tw, double2* qs, O* out) {
+    const int64_t L = n - m + 1;
+    const int lg = fft_log(m);
+    const int64_t h = int64_t(1) << (lg - 1);
+    int rc = std::is_same<S, float>::value
+                 ? launch_stats_f64out(st, reinterpret_cast<const float*>(s), n, m, avg, var)
+                 : launch_stats_d_var(st, reinterpret_cast<const double*>(s), n, m, avg, var);
+    if (rc != MDTW_OK) return rc;
+    hipLaunchKernelGGL(twiddle_kernel, dim3(static_cast<unsigned>((h + 255) / 256)), dim3(256), 0,
+                       st, tw, lg);
+    rc = hip_ok(hipGetLastError());
+    if (rc != MDTW_OK) return rc;
+    FftScalars a;
+    a.L = L;
+    a.n = n;
+    a.V = (int64_t(1) << lg) - m + 1;
+    a.m = m;
+    a.pad = 0;
+    a.q2 = q2;
+    a.qsum = qsum;
+    switch (lg) {
+        case 12: return launch_fft_pair<11>(st, s, avg, var, tw, q, qs, out, a);
+        case 13: return launch_fft_pair<12>(st, s, avg, var, tw, q, qs, out, a);
+        default: return launch_fft_pair<13>(st, s, avg, var, tw, q, qs, out, a);
+    }
 }
 
 constexpr int64_t kFftSeedTarget = 8192;
@@ -334,25 +374,7 @@ int enqueue_ed_fft(hipStream_t st, const float* s, int64_t n, const float* q, in
     double2* qs = cv.take<double2>(h + 1);
     if (!out) out = prof;
     if (prof_out) *prof_out = out;
-    int rc = launch_stats_f64out(st, s, n, m, avg, var);
-    if (rc != MDTW_OK) return rc;
-    hipLaunchKernelGGL(twiddle_kernel, dim3(static_cast<unsigned>((h + 255) / 256)), dim3(256), 0,
-                       st, tw, lg);
-    rc = hip_ok(hipGetLastError());
-    if (rc != MDTW_OK) return rc;
-    FftScalars a;
-    a.L = L;
-    a.n = n;
-    a.V = (int64_t(1) << lg) - m + 1;
-    a.m = m;
-    a.pad = 0;
-    a.q2 = q2;
-    a.qsum = qsum;
-    switch (lg) {
-        case 12: return launch_fft_pair<11>(st, s, avg, var, tw, q, qs, out, a);
-        case 13: return launch_fft_pair<12>(st, s, avg, var, tw, q, qs, out, a);
-        default: return launch_fft_pair<13>(st, s, avg, var, tw, q, qs, out, a);
-    }
+    return run_ed_fft<float, float>(st, s, n, q, m, q2, qsum, avg, var, tw, qs, out);
 }
 
 }  // namespace mdtw
@@ -493,3 +515,179 @@ int search_ed_fft(mdtw_engine* e, const float* d_series, int64_t n, int32_t stat
 }
 
 }  // namespace mdtw
+
+// ---- FP64 streams ------------------------------------------------------------
+// The reference's ED driver is FP64 end to end (GPU-ed.cu:20-82: an FP64
+// stream, avg_std<double>, calculate_ed on doubles, ed.cuh:71-153).  These
+// entry points take the FP64 stream and the FP64 z-normalised query as they
+// are: the same overlap-save transform with FP64 samples, FP64 window
+// statistics of the FP64 stream, an FP64 profile, and its argmin in FP64.
+namespace {
+
+__global__ void f64_min_reset_kernel(unsigned long long* r) {
+    if (threadIdx.x == 0) {
+        r[0] = ~0ull;
+        r[1] = ~0ull;
+    }
+}
+
+// r[0] = bits of the lowest value of v[0..n) (NaN skipped; the profile is
+// >= +0, where the bit patterns order like the values).
+__global__ __launch_bounds__(256) void f64_min_bits_kernel(const double* __restrict__ v,
+                                                           int64_t n, unsigned long long* r) {
+    unsigned long long b = ~0ull;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double x = v[i];
+        const unsigned long long k = static_cast<unsigned long long>(__double_as_longlong(x));
+        if (x == x && k < b) b = k;
+    }
+    b = wave_min_u64(b);
+    if ((threadIdx.x & (kWave - 1)) == 0 && b != ~0ull) atomicMin(&r[0], b);
+}
+
+// r[1] = the lowest index holding that value.
+__global__ __launch_bounds__(256) void f64_min_index_kernel(const double* __restrict__ v,
+                                                            int64_t n, unsigned long long* r) {
+    const unsigned long long want = r[0];
+    unsigned long long b = ~0ull;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const unsigned long long k = static_cast<unsigned long long>(__double_as_longlong(v[i]));
+        if (k == want && static_cast<unsigned long long>(i) < b) b = static_cast<unsigned long long>(i);
+    }
+    b = wave_min_u64(b);
+    if ((threadIdx.x & (kWave - 1)) == 0 && b != ~0ull) atomicMin(&r[1], b);
+}
+
+int check_f64_query(const double* h_zquery, int32_t m) {
+    if (!h_zquery || m < 1) return MDTW_ERR_INVALID_ARGUMENT;
+    if (m > MDTW_MAX_QUERY || fft_log(m) > kFftMaxLog) return MDTW_ERR_QUERY_TOO_LONG;
+    for (int32_t i = 0; i < m; ++i)
+        if (!std::isfinite(h_zquery[i])) return MDTW_ERR_NOT_FINITE;
+    return MDTW_OK;
+}
+
+size_t ed_f64_ws_bytes(int64_t L, int32_t m) {
+    const int64_t h = int64_t(1) << (fft_log(m) - 1);
+    return carve_size({sizeof(double) * L, sizeof(double) * L, sizeof(double) * L,
+                       sizeof(double2) * h, sizeof(double2) * (h + 1), sizeof(double) * m,
+                       2 * sizeof(unsigned long long)});
+}
+
+struct F64Bufs {
+    double *avg, *var, *prof;
+    double2 *tw, *qs;
+    double* q;
+    unsigned long long* r;
+};
+
+// Uploads the query and enqueues the profile (into out, or the workspace's).
+int enqueue_ed_f64(mdtw_engine* e, const double* d_series, int64_t n, const double* h_zquery,
+                   int32_t m, char* ws, double* out, F64Bufs* b) {
+    const int64_t L = n - m + 1;
+    const int64_t h = int64_t(1) << (fft_log(m) - 1);
+    Carve cv{ws};
+    b->avg = cv.take<double>(L);
+    b->var = cv.take<double>(L);
+    b->prof = cv.take<double>(L);
+    b->tw = cv.take<double2>(h);
+    b->qs = cv.take<double2>(h + 1);
+    b->q = cv.take<double>(m);
+    b->r = cv.take<unsigned long long>(2);
+    if (out) b->prof = out;
+    double q2 = 0.0, qsum = 0.0;
+    for (int32_t i = 0; i < m; ++i) {
+        q2 += h_zquery[i] * h_zquery[i];
+        qsum += h_zquery[i];
+    }
+    int rc = upload(e, b->q, h_zquery, sizeof(double) * m);
+    if (rc != MDTW_OK) return rc;
+    return run_ed_fft<double, double>(e->stream, d_series, n, b->q, m, q2, qsum, b->avg, b->var,
+                                      b->tw, b->qs, b->prof);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mdtw_avg_std_f64(mdtw_engine* e, const double* d_series, int64_t n, int32_t m,
+                     double* d_avg, double* d_std) {
+    if (!e || !d_series || !d_avg || !d_std || m < 1) return MDTW_ERR_INVALID_ARGUMENT;
+    if (m > MDTW_MAX_QUERY) return MDTW_ERR_QUERY_TOO_LONG;
+    if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
+    DeviceGuard guard(e->device);
+    return launch_stats_d(e->stream, d_series, n, m, d_avg, d_std, true);
+}
+
+int mdtw_ed_f64(mdtw_engine* e, const double* d_series, int64_t n, const double* h_zquery,
+                int32_t m, double* d_out) {
+    if (!e || !d_series || !d_out) return MDTW_ERR_INVALID_ARGUMENT;
+    int rc = check_f64_query(h_zquery, m);
+    if (rc != MDTW_OK) return rc;
+    if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
+    DeviceGuard guard(e->device);
+    void* ws = nullptr;
+    rc = ws_reserve(e, ed_f64_ws_bytes(n - m + 1, m), &ws);
+    if (rc != MDTW_OK) return rc;
+    F64Bufs b;
+    return enqueue_ed_f64(e, d_series, n, h_zquery, m, static_cast<char*>(ws), d_out, &b);
+}
+
+int mdtw_ed_search_f64(mdtw_engine* e, const double* d_series, int64_t n,
+                       const double* h_zquery, int32_t m, int64_t* h_pos, double* h_dist,
+                       mdtw_search_info* info) {
+    if (!e || !d_series || !h_pos || !h_dist) return MDTW_ERR_INVALID_ARGUMENT;
+    int rc = check_f64_query(h_zquery, m);
+    if (rc != MDTW_OK) return rc;
+    if (n < m) return MDTW_ERR_STREAM_TOO_SHORT;
+    const int64_t L = n - m + 1;
+    if (L > kMaxSearchWindows) return MDTW_ERR_UNSUPPORTED;
+    *h_pos = -1;
+    *h_dist = INFINITY;
+    DeviceGuard guard(e->device);
+    void* ws = nullptr;
+    rc = ws_reserve(e, ed_f64_ws_bytes(L, m), &ws);
+    if (rc != MDTW_OK) return rc;
+    hipStream_t st = e->stream;
+    F64Bufs b;
+    rc = hip_ok(hipEventRecord(e->ev0, st));
+    if (rc == MDTW_OK)
+        rc = enqueue_ed_f64(e, d_series, n, h_zquery, m, static_cast<char*>(ws), nullptr, &b);
+    const unsigned nbl = static_cast<unsigned>(
+        std::min<int64_t>((L + 255) / 256, static_cast<int64_t>(e->cus) * 16));
+    if (rc == MDTW_OK) {
+        hipLaunchKernelGGL(f64_min_reset_kernel, dim3(1), dim3(64), 0, st, b.r);
+        hipLaunchKernelGGL(f64_min_bits_kernel, dim3(nbl), dim3(256), 0, st, b.prof, L, b.r);
+        hipLaunchKernelGGL(f64_min_index_kernel, dim3(nbl), dim3(256), 0, st, b.prof, L, b.r);
+        rc = hip_ok(hipGetLastError());
+    }
+    unsigned long long r[2] = {~0ull, ~0ull};
+    if (rc == MDTW_OK) rc = hip_ok(hipEventRecord(e->ev1, st));
+    if (rc == MDTW_OK)
+        rc = hip_ok(hipMemcpyAsync(r, b.r, sizeof(r), hipMemcpyDeviceToHost, st));
+    if (rc == MDTW_OK) rc = hip_ok(hipStreamSynchronize(st));
+    if (rc != MDTW_OK) return rc;
+    double best = INFINITY;
+    if (r[0] != ~0ull && r[1] != ~0ull) {
+        std::memcpy(&best, &r[0], sizeof(best));
+        *h_pos = static_cast<int64_t>(r[1]);
+        *h_dist = std::sqrt(best);
+    }
+    if (info) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
+        info->windows = L;
+        info->seeded = 0;
+        info->lb_kim_pruned = 0;
+        info->lb_keogh_pruned = 0;
+        info->dp_evaluated = L;
+        info->dp_abandoned = 0;
+        info->found = *h_pos >= 0 ? 1 : 0;
+        info->threshold = static_cast<float>(best);
+        info->elapsed_ms = ms;
+    }
+    return MDTW_OK;
+}
+
+}  // extern "C"

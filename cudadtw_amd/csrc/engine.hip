@@ -13,6 +13,7 @@ static thread_local hipError_t g_last_hip = hipSuccess;
 void set_last_hip_error(hipError_t e) { g_last_hip = e; }
 
 int ws_reserve(mdtw_engine* e, size_t bytes, void** out) {
+    if (e->ws_limit && bytes > e->ws_limit) return MDTW_ERR_WORKSPACE;
     if (bytes <= e->ws_cap && e->d_ws) {
         *out = e->d_ws;
         return MDTW_OK;
@@ -23,6 +24,7 @@ int ws_reserve(mdtw_engine* e, size_t bytes, void** out) {
         return MDTW_ERR_HIP;
     }
     size_t cap = std::max(bytes, e->ws_cap + e->ws_cap / 2);
+    if (e->ws_limit) cap = std::min(cap, e->ws_limit);
     if (e->d_ws) (void)hipFree(e->d_ws);
     e->d_ws = nullptr;
     e->ws_cap = 0;
@@ -217,6 +219,12 @@ int mdtw_destroy(mdtw_engine* e) {
     return MDTW_OK;
 }
 
+int mdtw_set_workspace_limit(mdtw_engine* e, size_t bytes) {
+    if (!e) return MDTW_ERR_INVALID_ARGUMENT;
+    e->ws_limit = bytes;
+    return MDTW_OK;
+}
+
 int mdtw_set_stream(mdtw_engine* e, void* hip_stream) {
     if (!e) return MDTW_ERR_INVALID_ARGUMENT;
     e->stream = static_cast<hipStream_t>(hip_stream);
@@ -316,7 +324,8 @@ int mdtw_avg_std(mdtw_engine* e, const float* d_series, int64_t n, int32_t m, fl
     if (mode == MDTW_STATS_FP64) return launch_stats_fp64(e->stream, d_series, n, m, d_avg, d_std);
     if (mode == MDTW_STATS_REFCOMPAT)
         return launch_stats_refcompat(e, d_series, n, m, d_avg, d_std);
-    if (mode == MDTW_STATS_NONE) return launch_stats_none(e->stream, n - m + 1, d_avg, d_std);
+    if (mode == MDTW_STATS_NONE)
+        return launch_stats_none(e->stream, d_series, n, m, d_avg, d_std);
     return MDTW_ERR_INVALID_ARGUMENT;
 }
 
@@ -401,7 +410,7 @@ int mdtw_ldtw(mdtw_engine* e, const float* d_series, const float* d_avg, const f
         Carve cv{static_cast<char*>(ws)};
         float* avg = cv.take<float>(count);
         float* sd = cv.take<float>(count);
-        rc = launch_stats_none(e->stream, count, avg, sd);
+        rc = launch_stats_none(e->stream, d_series, count + e->m - 1, e->m, avg, sd);
         if (rc != MDTW_OK) return rc;
         d_avg = avg;
         d_std = sd;

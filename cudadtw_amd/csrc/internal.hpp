@@ -84,9 +84,11 @@ struct mdtw_engine {
     int32_t qx_len = 0;
     int32_t w_qx = -1;
 
-    // Growing scratch, reused across calls (no per-call malloc).
+    // Growing scratch, reused across calls (no per-call malloc), and the cap
+    // its growth may not pass (0: none; mdtw_set_workspace_limit).
     char* d_ws = nullptr;
     size_t ws_cap = 0;
+    size_t ws_limit = 0;
 
     mdtw::Ctrl* d_ctrl = nullptr;
     mdtw::Ctrl* h_ctrl = nullptr;  // pinned
@@ -312,8 +314,19 @@ int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m,
 // tile sums as launch_stats_fp64, kept in FP64.
 int launch_stats_f64out(hipStream_t st, const float* s, int64_t n, int32_t m, double* avg,
                         double* var);
-// MDTW_STATS_NONE: avg[0..count) = 0, sd[0..count) = 1.
-int launch_stats_none(hipStream_t st, int64_t count, float* avg, float* sd);
+// MDTW_STATS_NONE: avg = 0, sd = 1 for the n - m + 1 windows of s (NaN for a
+// window holding a non-finite sample).
+int launch_stats_none(hipStream_t st, const float* s, int64_t n, int32_t m, float* avg,
+                      float* sd);
+// FP64 stream: FP64 mean and population sigma (1 for a constant window, NaN
+// for a window holding a non-finite sample) of the n - m + 1 windows; with
+// sigma false the variance instead (0 for a constant window).
+int launch_stats_d(hipStream_t st, const double* s, int64_t n, int32_t m, double* avg,
+                   double* sd, bool sigma = true);
+inline int launch_stats_d_var(hipStream_t st, const double* s, int64_t n, int32_t m, double* avg,
+                              double* var) {
+    return launch_stats_d(st, s, n, m, avg, var, false);
+}
 int launch_stats_refcompat(mdtw_engine* e, const float* s, int64_t n, int32_t m,
                            float* avg, float* sd);
 // MDTW_STATS_FP64 statistics of listed windows (window pos[p], length len[p]),

@@ -219,7 +219,7 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
                                  tscratch, topk ? static_cast<int64_t>(tighten_ws_words()) : 0);
     if (rc == MDTW_OK && own_stats) {
         rc = p.stats_mode == MDTW_STATS_REFCOMPAT ? launch_stats_refcompat(p.e, p.s, p.n, m, avg, sd)
-             : p.stats_mode == MDTW_STATS_NONE  ? launch_stats_none(st, p.n - m + 1, avg, sd)
+             : p.stats_mode == MDTW_STATS_NONE  ? launch_stats_none(st, p.s, p.n, m, avg, sd)
                                                 : launch_stats_fp64(st, p.s, p.n, m, avg, sd);
     }
     if (rc != MDTW_OK) return rc;

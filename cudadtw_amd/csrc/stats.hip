@@ -14,6 +14,13 @@
 // HBM traffic: read (2T + m)/T floats per window (the second read of each
 // sample mostly hits L2), write 8 bytes.
 //
+// Non-finite samples (NaN, +-Inf), FP64 and raw modes: a window holding one
+// gets NaN for its mean and sigma, which makes every distance and bound of it
+// NaN (unordered: never a match, never kept by a top-k bound).  The tile sums
+// skip such samples (counted instead), so they never reach the windows
+// around them; without that, one NaN would poison every later window of its
+// tile through the running sums.
+//
 // MDTW_STATS_REFCOMPAT.  Restates the reference arithmetic: per 10^7-sample
 // epoch with M-1 overlap (stats.cuh:120-141), a global inclusive FP64 prefix
 // with P[0] = 0 written over the epoch's first sample, which the in-place
@@ -66,18 +73,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     for (int i = 0; i < kThreads / kWave; ++i) t += sh[i];
     __syncthreads();
     return t;
-}
-
-// ---- MDTW_STATS_NONE: raw windows ------------------------------------------
-// mean 0 and sigma 1 make (x - mean) / sigma == x exactly in every kernel.
-__global__ __launch_bounds__(256) void stats_none_kernel(int64_t count, float* __restrict__ avg,
-                                                         float* __restrict__ sd) {
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < count;
-         i += stride) {
-        avg[i] = 0.0f;
-        sd[i] = 1.0f;
-    }
 }
 
 // ---- MDTW_STATS_FP64: wave-local tiles -------------------------------------
@@ -142,19 +137,24 @@ __device__ __forceinline__ void stats_wave_fence() {
 // whole-stream kernel and the window-list kernel below both run exactly this,
 // so a window's statistics do not depend on which of them computed it.
 // T = double (the FFT ED's statistics): the same sums, leaving the FP64 mean
-// and the FP64 variance (0 for a constant window) instead.
-template <class T = float>
-__device__ __forceinline__ int stats_tile(const float* __restrict__ s, int64_t L, int32_t m,
+// and the FP64 variance (0 for a constant window) instead.  S is the sample
+// type (FP32 streams, or FP64 ones for the FP64 ED; T must hold S exactly).
+// RAW (MDTW_STATS_NONE): mean 0 and sigma 1, which make (x - mean) / sigma
+// == x exactly in every kernel -- only the non-finite rule is computed.
+// A window holding a non-finite sample gets NaN mean and sigma (variance).
+template <class T = float, class S = float, bool RAW = false>
+__device__ __forceinline__ int stats_tile(const S* __restrict__ s, int64_t L, int32_t m,
                                           int64_t w0, T* xa, T* xb) {
+    static_assert(sizeof(T) >= sizeof(S), "the LDS tile must hold samples exactly");
     const int lane = threadIdx.x & (kWave - 1);
     const int nw = L - w0 < kStatsTW ? static_cast<int>(L - w0) : kStatsTW;
     const int64_t last = L - 1 + m - 1;  // last stream sample
-    const float* x = s + w0;
-    const double c = static_cast<double>(x[0]);
+    const S* x = s + w0;
+    const double c = isfinite(x[0]) ? static_cast<double>(x[0]) : 0.0;
 
     // Stage x[w0 + i] (leaving sample of window i) and x[w0 + m + i] (entering);
     // every load of the tile is issued before any is used.
-    float va[kStatsR], vb[kStatsR];
+    S va[kStatsR], vb[kStatsR];
 #pragma unroll
     for (int v = 0; v < kStatsR; ++v) {
         const int i = v * kWave + lane;
@@ -172,36 +172,48 @@ __device__ __forceinline__ int stats_tile(const float* __restrict__ s, int64_t L
 
     // First window of the tile: x[w0 .. w0 + m), from the staged tile where
     // it lies inside it (x[w0 + k] = xa[k] for k < kStatsTW).
-    double a1 = 0.0, a2 = 0.0, b1 = 0.0;
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, f1 = 0.0;
     for (int k = lane; k < m; k += kWave) {
-        const float xk = k < kStatsTW ? xa[stats_skew(k)] : x[k];
-        const double d = static_cast<double>(xk) - c;
+        const S xk = k < kStatsTW ? static_cast<S>(xa[stats_skew(k)]) : x[k];
+        const bool fin = isfinite(xk);
+        const double d = fin ? static_cast<double>(xk) - c : 0.0;
         a1 += d;
         b1 += fabs(d);
         a2 = fma(d, d, a2);
+        f1 += fin ? 0.0 : 1.0;
     }
     const double S1 = wave_sum_f64(a1);
     const double S2 = wave_sum_f64(a2);
     const double B1 = wave_sum_f64(b1);
+    const double F1 = wave_sum_f64(f1);  // non-finite samples of the first window
 
     // Pass 1: this lane's transition totals (window r -> r + 1).
     const int r0 = lane * kStatsR;
-    double t1 = 0.0, t2 = 0.0, u1 = 0.0, u2 = 0.0;
+    // A non-finite sample enters the sums as 0 and the count as 1.
+    auto shifted = [c](T v, double& fcount) {
+        const bool fin = isfinite(v);
+        fcount += fin ? 0.0 : 1.0;
+        return fin ? static_cast<double>(v) - c : 0.0;
+    };
+    double t1 = 0.0, t2 = 0.0, u1 = 0.0, u2 = 0.0, tf = 0.0;
     for (int v = 0; v < kStatsR; ++v) {
         const int r = r0 + v;
         if (r + 1 < nw) {
-            const double xo = static_cast<double>(xa[stats_skew(r)]) - c;
-            const double xn = static_cast<double>(xb[stats_skew(r)]) - c;
+            double fo = 0.0, fn = 0.0;
+            const double xo = shifted(xa[stats_skew(r)], fo);
+            const double xn = shifted(xb[stats_skew(r)], fn);
             t1 += xn - xo;
             t2 += xn * xn - xo * xo;
             u1 += fabs(xn) + fabs(xo);
             u2 += xn * xn + xo * xo;
+            tf += fn - fo;
         }
     }
     double p1 = wave_excl_scan_f64(t1);
     double p2 = wave_excl_scan_f64(t2);
     double q1 = wave_excl_scan_f64(u1);
     double q2 = wave_excl_scan_f64(u2);
+    double pf = wave_excl_scan_f64(tf);
 
     // Pass 2: the lane's windows; results overwrite their own LDS slots
     // after the slot's samples are read.
@@ -220,21 +232,28 @@ __device__ __forceinline__ int stats_tile(const float* __restrict__ s, int64_t L
         const double tol = 64.0 * DBL_EPSILON * (mag2 + 2.0 * fabs(mean) * mag1);
         if (var <= tol) var = 0.0;
         const int k = stats_skew(r);
-        const double xo = static_cast<double>(xa[k]) - c;
-        const double xn = static_cast<double>(xb[k]) - c;
-        if constexpr (sizeof(T) == sizeof(double)) {
-            xa[k] = c + mean;
-            xb[k] = var;
+        const bool bad = F1 + pf != 0.0;  // the window holds a non-finite sample
+        double fo = 0.0, fn = 0.0;
+        const double xo = shifted(xa[k], fo);
+        const double xn = shifted(xb[k], fn);
+        const T nan = static_cast<T>(__builtin_nan(""));
+        if constexpr (RAW) {
+            xa[k] = bad ? nan : static_cast<T>(0);
+            xb[k] = bad ? nan : static_cast<T>(1);
+        } else if constexpr (sizeof(T) == sizeof(double)) {
+            xa[k] = bad ? nan : c + mean;
+            xb[k] = bad ? nan : var;
         } else {
-            xa[k] = static_cast<float>(c + mean);
+            xa[k] = bad ? nan : static_cast<float>(c + mean);
             // FP32 square root of the FP64 variance rounded to FP32: within
             // one ulp of the FP64 root's rounding, at a fraction of its cost.
-            xb[k] = var > 0.0 ? sqrtf(static_cast<float>(var)) : 1.0f;
+            xb[k] = bad ? nan : (var > 0.0 ? sqrtf(static_cast<float>(var)) : 1.0f);
         }
         p1 += xn - xo;
         p2 += xn * xn - xo * xo;
         q1 += fabs(xn) + fabs(xo);
         q2 += xn * xn + xo * xo;
+        pf += fn - fo;
     }
     stats_wave_fence();
     return nw;
@@ -281,6 +300,59 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_f64out_kernel(
         if (i < nw) {
             __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
             __builtin_nontemporal_store(xb[stats_skew(i)], var + w0 + i);
+        }
+    }
+}
+
+// MDTW_STATS_NONE: mean 0 and sigma 1 for every window (NaN for a window
+// holding a non-finite sample).
+__global__ __launch_bounds__(kStatsWaves * kWave) void stats_none_kernel(
+    const float* __restrict__ s, int64_t L, int32_t m, float* __restrict__ avg,
+    float* __restrict__ sd) {
+    __shared__ float xa_all[kStatsWaves][kStatsLds];
+    __shared__ float xb_all[kStatsWaves][kStatsLds];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    float* xa = xa_all[wid];
+    float* xb = xb_all[wid];
+    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
+    if (w0 >= L) return;
+    const int nw = stats_tile<float, float, true>(s, L, m, w0, xa, xb);
+#pragma unroll 4
+    for (int v = 0; v < kStatsR; ++v) {
+        const int i = v * kWave + lane;
+        if (i < nw) {
+            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
+            __builtin_nontemporal_store(xb[stats_skew(i)], sd + w0 + i);
+        }
+    }
+}
+
+// FP64 stream (the reference's avg_std<double> for its ED driver): FP64 mean
+// and population sigma of every window, sigma := 1 for a constant window
+// (SIGMA), or the variance, 0 for a constant window (the FFT ED's input).
+template <bool SIGMA>
+__global__ __launch_bounds__(kStatsWaves * kWave) void stats_d_kernel(
+    const double* __restrict__ s, int64_t L, int32_t m, double* __restrict__ avg,
+    double* __restrict__ sd) {
+    __shared__ double xa_all[kStatsWaves][kStatsLds];
+    __shared__ double xb_all[kStatsWaves][kStatsLds];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    double* xa = xa_all[wid];
+    double* xb = xb_all[wid];
+    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
+    if (w0 >= L) return;
+    const int nw = stats_tile<double, double>(s, L, m, w0, xa, xb);
+#pragma unroll 4
+    for (int v = 0; v < kStatsR; ++v) {
+        const int i = v * kWave + lane;
+        if (i < nw) {
+            const double var = xb[stats_skew(i)];
+            // NaN (a non-finite sample) stays NaN
+            const double sg = !SIGMA ? var : (var > 0.0 ? sqrt(var) : (var == 0.0 ? 1.0 : var));
+            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
+            __builtin_nontemporal_store(sg, sd + w0 + i);
         }
     }
 }
@@ -459,12 +531,34 @@ int launch_stats_f64out(hipStream_t st, const float* s, int64_t n, int32_t m, do
     return MDTW_OK;
 }
 
-int launch_stats_none(hipStream_t st, int64_t count, float* avg, float* sd) {
-    if (count <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
-    constexpr int64_t per_block = 256 * 4;
-    const int64_t nb = std::min<int64_t>((count + per_block - 1) / per_block, 65536);
-    hipLaunchKernelGGL(stats_none_kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, st,
-                       count, avg, sd);
+int launch_stats_none(hipStream_t st, const float* s, int64_t n, int32_t m, float* avg,
+                      float* sd) {
+    const int64_t L = n - m + 1;
+    if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
+    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
+    const int64_t nb = (L + per_block - 1) / per_block;
+    hipLaunchKernelGGL(stats_none_kernel, dim3(static_cast<unsigned>(nb)),
+                       dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
+    hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        set_last_hip_error(err);
+        return MDTW_ERR_HIP;
+    }
+    return MDTW_OK;
+}
+
+int launch_stats_d(hipStream_t st, const double* s, int64_t n, int32_t m, double* avg,
+                   double* sd, bool sigma) {
+    const int64_t L = n - m + 1;
+    if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
+    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
+    const int64_t nb = (L + per_block - 1) / per_block;
+    if (sigma)
+        hipLaunchKernelGGL(stats_d_kernel<true>, dim3(static_cast<unsigned>(nb)),
+                           dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
+    else
+        hipLaunchKernelGGL(stats_d_kernel<false>, dim3(static_cast<unsigned>(nb)),
+                           dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

@@ -11,14 +11,22 @@
  *
  * Conventions
  *   - d_* arguments are device pointers (HBM), h_* are host pointers.
- *   - Stream data is FP32; window statistics are FP32 (mean, population
- *     sigma), computed from FP64 prefix sums.
+ *   - Stream data is FP32 (the *_f64 ED entry points: FP64, as the
+ *     reference's ED driver); window statistics are FP32 (mean, population
+ *     sigma), computed from FP64 sums.
+ *   - Non-finite samples (NaN, +-Inf) in the stream: a window holding one has
+ *     NaN statistics (FP64 and raw modes) and so NaN distances and bounds --
+ *     never a match, never kept by top-k; other windows are unaffected.
+ *     REFCOMPAT restates the reference, whose running prefix carries such a
+ *     sample into every later window of its epoch.  Queries must be finite
+ *     (MDTW_ERR_NOT_FINITE).
  *   - Window i is series[i .. i+M); there are L = N - M + 1 windows.
  *   - DP outputs are squared cDTW distances, as the reference's Cdtw array
  *     (cdtw.cuh:256); reported search distances are sqrt of that
  *     (cdtw.cuh:507).  Ties go to the lowest window index.
  *   - Index arrays on the device are int32 (the reference's index_t = int);
- *     host-facing positions are int64.
+ *     host-facing positions are int64.  One search covers at most 2^31 - 1
+ *     windows (MDTW_ERR_UNSUPPORTED beyond; streams: per epoch).
  *   - One engine per host thread.  Calls on one engine are ordered on its
  *     stream; distinct engines may run concurrently.
  */
@@ -98,6 +106,13 @@ int mdtw_destroy(mdtw_engine* e);
  * in places (stats.cuh:104-108, ed.cuh:111). */
 int mdtw_set_stream(mdtw_engine* e, void* hip_stream);
 int mdtw_synchronize(mdtw_engine* e);
+/* No reference counterpart (the reference mallocs per call, cdtw.cuh:510-592).
+ * Caps the device workspace the engine may grow to (bytes; 0 = no cap): a call
+ * whose scratch would exceed it returns MDTW_ERR_WORKSPACE before any launch
+ * and leaves the engine usable.  A workspace already larger is kept; the cap
+ * applies to growth and to each call's need.  Batches and streams allocate
+ * their own buffers at creation and are not affected. */
+int mdtw_set_workspace_limit(mdtw_engine* e, size_t bytes);
 const char* mdtw_status_string(int status);
 /* Device name, compute-unit count; for logs. */
 int mdtw_device_info(mdtw_engine* e, char* name, int32_t name_len, int32_t* cus);
@@ -159,6 +174,26 @@ int mdtw_ed(mdtw_engine* e, const float* d_series, const float* d_avg,
  * 8192; faster than mdtw_ed for long queries. */
 int mdtw_ed_fft(mdtw_engine* e, const float* d_series, int64_t n,
                 int32_t stats_mode, float* d_out);
+
+/* ---- FP64 streams (the reference's ED driver, GPU-ed.cu:20-82) ----------- */
+/* stats.cuh:115-142 avg_std<double> on an FP64 stream: FP64 mean and
+ * population sigma of every window (sigma := 1 for a constant window), from
+ * FP64 tile-local sums. */
+int mdtw_avg_std_f64(mdtw_engine* e, const double* d_series, int64_t n, int32_t m,
+                     double* d_avg, double* d_std);
+/* ed.cuh:71-153 calculate_ed's profile on an FP64 stream: d_out[n-M+1] FP64
+ * squared z-normalised ED of every window against h_zquery (M FP64 values,
+ * z-normalised with mdtw_znormalize_f64; M <= 8192), through FP64 FFT
+ * cross-correlation with FP64 window statistics computed here. */
+int mdtw_ed_f64(mdtw_engine* e, const double* d_series, int64_t n, const double* h_zquery,
+                int32_t m, double* d_out);
+/* ed.cuh:71-153 calculate_ed on an FP64 stream: the best window (lowest index
+ * on ties) and its distance sqrt(ED^2), both FP64-computed (the reference
+ * prints sqrt(2M + bsf) and the location, ed.cuh:149-150).  At most 2^31 - 1
+ * windows. */
+int mdtw_ed_search_f64(mdtw_engine* e, const double* d_series, int64_t n,
+                       const double* h_zquery, int32_t m, int64_t* h_pos, double* h_dist,
+                       mdtw_search_info* info);
 
 /* No reference counterpart: the same profile as mdtw_cdtw, forced through the
  * anti-diagonal kernel that serves bands wider than the register-band kernel

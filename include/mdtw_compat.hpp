@@ -85,6 +85,18 @@ inline void set_query(const float* zquery, const float* zlower, const float* zup
     check(mdtw_set_query(engine(), zquery, zlower, zupper, M, W), "mdtw_set_query");
 }
 
+// Replaces cudaMemcpyToSymbol(::Czquery, zquery, sizeof(double) * M)
+// (GPU-ed.cu:72): the FP64 z-normalised query of the ED driver, kept here for
+// calculate_ed on FP64 streams.
+inline std::vector<double>& query_f64() {
+    static std::vector<double> q;
+    return q;
+}
+inline void set_query(const double* zquery, int M) {
+    if (!zquery || M < 1) throw error(MDTW_ERR_INVALID_ARGUMENT, "set_query");
+    query_f64().assign(zquery, zquery + M);
+}
+
 struct match {
     int64_t location;
     double distance;
@@ -92,6 +104,8 @@ struct match {
 
 template <class T>
 using req_float = typename std::enable_if<std::is_same<T, float>::value, int>::type;
+template <class T>
+using req_double = typename std::enable_if<std::is_same<T, double>::value, int>::type;
 
 }  // namespace mdtw_compat
 
@@ -127,6 +141,18 @@ inline void avg_std(const value_t* const Series, value_t* const AvgS, value_t* c
     mdtw_compat::check(mdtw_avg_std(mdtw_compat::on_stream(stream), Series,
                                     static_cast<int64_t>(N), static_cast<int32_t>(M), AvgS, StdS,
                                     mode),
+                       "avg_std");
+}
+
+// stats.cuh:115-142 avg_std<double> on an FP64 stream (value_t = double, the
+// ED driver GPU-ed.cu:76): FP64 window means and population sigmas.
+template <class prefx_t, class value_t, class index_t, mdtw_compat::req_double<value_t> = 0>
+inline void avg_std(const value_t* const Series, value_t* const AvgS, value_t* const StdS,
+                    const index_t M, const index_t N, void* stream = nullptr) {
+    static_assert(std::is_same<prefx_t, double>::value, "avg_std: prefix sums are FP64");
+    mdtw_compat::check(mdtw_avg_std_f64(mdtw_compat::on_stream(stream), Series,
+                                        static_cast<int64_t>(N), static_cast<int32_t>(M), AvgS,
+                                        StdS),
                        "avg_std");
 }
 
@@ -253,6 +279,30 @@ inline double calculate_ed(const value_t* const Subject, const value_t* const St
     std::cout << "distance: " << ed << std::endl;
     std::cout << "location: " << pos << std::endl;
     return ed * ed - 2.0 * static_cast<double>(M);
+}
+
+// ed.cuh:71-153 calculate_ed on an FP64 stream, as the reference's ED driver
+// calls it (GPU-ed.cu:82): the FP64 query given to set_query(const double*,
+// M), FP64 window statistics (computed here; StdS is accepted for signature
+// parity and not read), FP64 cross-correlation.  Prints the reference's
+// result lines and returns its bsf = ED^2 - 2M.
+template <class value_t, class index_t, mdtw_compat::req_double<value_t> = 0>
+inline double calculate_ed(const value_t* const Subject, const value_t* const StdS,
+                           const index_t M, const index_t N, void* stream = nullptr) {
+    (void)StdS;
+    const std::vector<double>& q = mdtw_compat::query_f64();
+    if (static_cast<index_t>(q.size()) != M)
+        throw mdtw_compat::error(MDTW_ERR_NO_QUERY, "calculate_ed");
+    int64_t pos = -1;
+    double dist = INFINITY;
+    mdtw_compat::check(mdtw_ed_search_f64(mdtw_compat::on_stream(stream), Subject,
+                                          static_cast<int64_t>(N), q.data(),
+                                          static_cast<int32_t>(M), &pos, &dist, nullptr),
+                       "calculate_ed");
+    std::cout << "\n= result ===================================" << std::endl;
+    std::cout << "distance: " << dist << std::endl;
+    std::cout << "location: " << pos << std::endl;
+    return dist * dist - 2.0 * static_cast<double>(M);
 }
 
 // ---- ldtw.cuh -----------------------------------------------------------------

@@ -7,6 +7,8 @@
  */
 #include "oracle.h"
 
+#include <omp.h>
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -156,11 +158,22 @@ void or_stats_refcompat(const float* s, int64_t n, int32_t m, int64_t epoch,
     free(uppers);
 }
 
+/* Non-finite samples: the library's rule (not the reference's, whose running
+ * prefix spreads them), stated here as the oracle's: a window holding a NaN
+ * or an infinity has NaN statistics, hence NaN distances and bounds. */
 void or_stats_fp64(const float* s, int64_t n, int32_t m, float* avg, float* sd,
                    double* avg64, double* sd64) {
     int64_t L = n - m + 1;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < L; ++i) {
+        int bad = 0;
+        for (int32_t k = 0; k < m; ++k) bad |= !isfinite(s[i + k]);
+        if (bad) {
+            avg[i] = sd[i] = NAN;
+            if (avg64) avg64[i] = NAN;
+            if (sd64) sd64[i] = NAN;
+            continue;
+        }
         double sum = 0.0;
         for (int32_t k = 0; k < m; ++k) sum += (double)s[i + k];
         double mean = sum / (double)m;
@@ -560,6 +573,14 @@ void or_cdtw_grad(const double* zq, const float* s, const double* avg,
 static float ldtw_one(const float* q, const float* sub, float a, float sdv,
                       int32_t M, int32_t W, int l2, uint8_t* moves) {
     int i = 0, j = 0, step = 0;
+    /* NaN statistics (a non-finite sample): NaN, and no walk -- with NaN
+     * costs every comparison fails and the reference's loop would step past
+     * the window (ldtw.cuh:92-123). */
+    if (!(a == a) || !(sdv == sdv)) {
+        if (moves)
+            for (; step < 2 * M - 2; ++step) moves[step] = 3;
+        return NAN;
+    }
     float d = q[i] - (sub[j] - a) / sdv;
     d = l2 ? d * d : (d < 0 ? -d : d);
     while (i + 1 != M || j + 1 != M) {
@@ -617,6 +638,8 @@ void or_ldtw(const float* zq, const float* s, const float* avg, const float* sd,
  * z-normalisation of the window; the query is taken as given). */
 static float ldtw_raw_one(const float* q, const float* x, int32_t M, int32_t W, int l2) {
     int i = 0, j = 0;
+    for (int32_t k = 0; k < M; ++k)
+        if (!isfinite(x[k])) return NAN; /* as ldtw_one: a non-finite sample */
     float d = q[0] - x[0];
     d = l2 ? d * d : (d < 0 ? -d : d);
     while (i + 1 != M || j + 1 != M) {
@@ -699,8 +722,11 @@ static int cmp_dist_idx(const void* a, const void* b) {
 
 int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
                        int64_t* out_idx, float* out_dist) {
-    int64_t* ord = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
-    for (int64_t i = 0; i < n; ++i) ord[i] = i;
+    int64_t* ord = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t nn = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (d[i] == d[i]) ord[nn++] = i; /* NaN distances are never picked */
+    n = nn;
     g_sort_d = d;
     qsort(ord, (size_t)n, sizeof(int64_t), cmp_dist_idx);
     int32_t got = 0;
@@ -724,3 +750,54 @@ int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
     free(ord);
     return got;
 }
+
+/* ------------------------------------------------------------------------ */
+/* FP64 streams (the reference's ED driver, GPU-ed.cu:20-82): two-pass FP64
+ * window statistics (sigma := 1 for a constant window, NaN for a window
+ * holding a non-finite sample), and the z-normalised squared ED of every
+ * window, sum_k (q_k - (x_{i+k} - mu_i) / sigma_i)^2 (ed.cuh:61-69 computes
+ * the same quantity as 2M - 2 corr / sigma through FFTs). */
+void or_stats_f64d(const double* s, int64_t n, int32_t m, double* avg, double* sd) {
+    int64_t L = n - m + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < L; ++i) {
+        int bad = 0;
+        double sum = 0.0;
+        for (int32_t k = 0; k < m; ++k) {
+            bad |= !isfinite(s[i + k]);
+            sum += s[i + k];
+        }
+        if (bad) {
+            avg[i] = sd[i] = NAN;
+            continue;
+        }
+        const double mean = sum / (double)m;
+        double ss = 0.0;
+        for (int32_t k = 0; k < m; ++k) {
+            const double d = s[i + k] - mean;
+            ss += d * d;
+        }
+        const double var = ss / (double)m;
+        avg[i] = mean;
+        sd[i] = var > 0.0 ? sqrt(var) : 1.0;
+    }
+}
+
+void or_ed_f64d(const double* zq, const double* s, int64_t n, int32_t m, const double* avg,
+                const double* sd, double* out) {
+    int64_t L = n - m + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < L; ++i) {
+        double acc = 0.0;
+        for (int32_t k = 0; k < m; ++k) {
+            const double d = zq[k] - (s[i + k] - avg[i]) / sd[i];
+            acc += d * d;
+        }
+        out[i] = acc;
+    }
+}
+
+/* Host threads of the OpenMP loops above (the one-core timings of the CPU
+ * baseline, the reference's sequential_cdtw and z_stream_ldtw). */
+void or_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t or_max_threads(void) { return omp_get_max_threads(); }

@@ -146,6 +146,16 @@ int64_t or_argmin(const float* x, int64_t n);
 int32_t or_topk_greedy(const float* d, int64_t n, int32_t k, int64_t excl,
                        int64_t* out_idx, float* out_dist);
 
+/* FP64 streams: two-pass FP64 window statistics and the z-normalised squared
+ * ED of every window (ed.cuh:61-69's quantity, computed directly). */
+void or_stats_f64d(const double* s, int64_t n, int32_t m, double* avg, double* sd);
+void or_ed_f64d(const double* zq, const double* s, int64_t n, int32_t m, const double* avg,
+                const double* sd, double* out);
+
+/* OpenMP threads of the functions above (1: the reference's one-core loops). */
+void or_set_threads(int32_t n);
+int32_t or_max_threads(void);
+
 /* cascade.c: the UCR-suite search restated from its paper (KDD 2012), pinned
  * to UCR_DTW_BIN / UCR_ED_BIN compiled from the reference sources.  Best
  * window of query (z-normalised here) in data with band r; *dist gets the

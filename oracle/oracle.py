@@ -55,6 +55,10 @@ def lib() -> C.CDLL:
                                _U8]),
             "or_ldtw_raw": (None, [_F, _F, C.c_int32, _I, C.c_int64, C.c_int32, C.c_int32, _F]),
             "or_ed_f64": (None, [_D, _F, C.c_int64, C.c_int32, _D, _D, _D]),
+            "or_stats_f64d": (None, [_D, C.c_int64, C.c_int32, _D, _D]),
+            "or_set_threads": (None, [C.c_int32]),
+            "or_max_threads": (C.c_int32, []),
+            "or_ed_f64d": (None, [_D, _D, C.c_int64, C.c_int32, _D, _D, _D]),
             "or_argmin": (C.c_int64, [_F, C.c_int64]),
             "or_topk_greedy": (C.c_int32, [_F, C.c_int64, C.c_int32, C.c_int64, _L, _F]),
             "or_ucr_dtw": (C.c_int64, [_D, C.c_int64, _D, C.c_int32, C.c_int32, _D, _L]),
@@ -216,6 +220,65 @@ def ed_f64(zq64, s, avg64, sd64):
     out = np.empty(L, np.float64)
     lib().or_ed_f64(_d(zq64), _f(s), s.size, zq64.size, _d(avg64), _d(sd64), _d(out))
     return out
+
+
+def stats_f64d(s, m):
+    """FP64 window statistics of an FP64 stream (two-pass)."""
+    s = _c64(s)
+    L = s.size - m + 1
+    avg = np.empty(L, np.float64)
+    sd = np.empty(L, np.float64)
+    lib().or_stats_f64d(_d(s), s.size, int(m), _d(avg), _d(sd))
+    return avg, sd
+
+
+def ed_f64d(zq64, s, avg64, sd64):
+    """Squared z-normalised ED of every window of an FP64 stream."""
+    zq64, s, avg64, sd64 = _c64(zq64), _c64(s), _c64(avg64), _c64(sd64)
+    L = s.size - zq64.size + 1
+    out = np.empty(L, np.float64)
+    lib().or_ed_f64d(_d(zq64), _d(s), s.size, zq64.size, _d(avg64), _d(sd64), _d(out))
+    return out
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops (1 = the reference's one-core form)."""
+    lib().or_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().or_max_threads())
+
+
+def ucr_reference(binary, data, query, r=None):
+    """Runs UCR_DTW_BIN / UCR_ED_BIN as compiled from the reference sources
+    into oracle/_ref (never a shipped binary): FP64 files, argv
+    ``data query m [R] n``.  Returns (location, distance, seconds as the
+    program reports them: CPU time of its search loop), or None when the
+    program is not built here."""
+    import tempfile
+
+    exe = os.path.join(REF_DIR, binary)
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        dp, qp = os.path.join(td, "d.bin"), os.path.join(td, "q.bin")
+        _c64(data).tofile(dp)
+        _c64(query).tofile(qp)
+        args = [exe, dp, qp, str(query.size)] + ([str(int(r))] if r is not None else []) + \
+               [str(data.size)]
+        out = subprocess.run(args, capture_output=True, text=True, check=True).stdout
+    loc = dist = secs = None
+    for line in out.splitlines():
+        key, _, val = line.partition(":")
+        key = key.strip()
+        if key == "Location":
+            loc = int(val)
+        elif key == "Distance":
+            dist = float(val)
+        elif key == "Total Execution Time":
+            secs = float(val.split()[0])
+    return loc, dist, secs
 
 
 def argmin(x):

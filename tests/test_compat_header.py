@@ -15,7 +15,8 @@ CALLER = r'''
 #define EUCLIDEAN (true)
 #define MANHATTAN (false)
 void caller(float* Subject, float* AvgS, float* StdS, float* out, int* Indices,
-            float* zquery, float* zlower, float* zupper, double* q64, float* R) {
+            float* zquery, float* zlower, float* zupper, double* q64, float* R,
+            double* S64, double* A64, double* D64) {
     int M = 128, N = 1 << 20, W = M * (10 * 0.01);
     znormalize(zquery, M);
     znormalize(q64, M);
@@ -32,7 +33,11 @@ void caller(float* Subject, float* AvgS, float* StdS, float* out, int* Indices,
     mdtw_compat::ztsh2_ldtw<true>(Subject, M, N, AvgS, StdS, R, W);
     mdtw_compat::tsh2_ldtw<false>(Subject, M, N, R, W);
     determine_min<true>(R, N - M + 1);
-    (void)r; (void)bsf;
+    // the ED driver's FP64 form (GPU-ed.cu:62-82)
+    mdtw_compat::set_query(q64, M);
+    avg_std<double>(S64, A64, D64, M, N);
+    double bsf64 = calculate_ed(S64, D64, M, N);
+    (void)r; (void)bsf; (void)bsf64;
 }
 '''
 

@@ -469,3 +469,50 @@ def test_golden_ucr_locations(case):
         _, _, a64, s64 = O.stats_fp64(s, q.size, want64=True)
         ed = O.ed_f64(q64, s, a64, s64)
         assert int(np.argmin(ed)) == ref["ucr_ed"]["location"]
+
+
+# ---- FP64 streams and non-finite samples ----------------------------------------------
+def test_stats_f64d_and_ed_f64d_closed_forms():
+    rng = np.random.default_rng(8)
+    s = 7000.0 + np.cumsum(rng.standard_normal(3000)) * 0.01
+    m = 100
+    avg, sd = O.stats_f64d(s, m)
+    win = np.lib.stride_tricks.sliding_window_view(s, m)
+    np.testing.assert_allclose(avg, win.mean(axis=1), rtol=1e-14)
+    np.testing.assert_allclose(sd, win.std(axis=1), rtol=1e-10)
+    q = rng.standard_normal(m)
+    zq = (q - q.mean()) / q.std()
+    ed = O.ed_f64d(zq, s, avg, sd)
+    z = (win - avg[:, None]) / sd[:, None]
+    np.testing.assert_allclose(ed, ((zq[None, :] - z) ** 2).sum(axis=1), rtol=1e-12)
+    # the reference's form (ed.cuh:61-69): 2M - 2 corr / sigma, corr = q . x
+    ref_form = 2 * m - 2 * (win @ zq) / sd
+    np.testing.assert_allclose(ed, ref_form, rtol=1e-6, atol=1e-6 * m)
+    # constant windows: sigma := 1, the distance is sum q^2 = M
+    c = np.concatenate([np.full(200, 3.5), s[:500]])
+    a2, s2 = O.stats_f64d(c, m)
+    assert np.all(s2[:101] == 1.0) and np.all(a2[:101] == 3.5)
+
+
+def test_nonfinite_rule_in_the_oracle():
+    s = synth.random_walk(3000, 2)
+    m, w = 50, 5
+    for p, v in ((0, np.nan), (700, np.inf), (1500, -np.inf)):
+        s[p] = v
+    bad = np.zeros(s.size - m + 1, bool)
+    for p in (0, 700, 1500):
+        bad[max(0, p - m + 1):p + 1] = True
+    avg, sd, a64, s64 = O.stats_fp64(s, m, want64=True)
+    assert np.array_equal(np.isnan(avg), bad) and np.array_equal(np.isnan(s64), bad)
+    clean = s.copy()
+    clean[[0, 700, 1500]] = 0.0
+    ca, csd = O.stats_fp64(clean, m)
+    assert np.array_equal(avg[~bad], ca[~bad]) and np.array_equal(sd[~bad], csd[~bad])
+    zq = O.znormalize_f32(synth.random_walk(m, 3))
+    for prof in (O.cdtw(zq, s, avg, sd, w), O.ldtw(zq, s, avg, sd, w),
+                 O.ldtw_raw(zq, s, w), O.lb_kim(s, avg, sd, zq)):
+        assert np.array_equal(np.isnan(prof), bad)  # and the walk terminated
+    prof = O.cdtw(zq, s, avg, sd, w)
+    idx, d = O.topk_greedy(prof, 10, 20)
+    assert idx.size == 10 and not np.any(bad[idx]) and np.all(np.isfinite(d))
+    assert not bad[O.argmin(prof)]
