@@ -39,95 +39,62 @@ __device__ __forceinline__ float lpa(float d, float acc) {
         return fabsf(d) + acc;
 }
 
-// bounds.cuh:283-407: front 5x5 unconstrained DTW, minimum over its exit
-// boundary, plus the back 5x5 block entered anywhere on its top row / left
-// column.  Needs m >= 11 (bounds.cuh:418).
+// bounds.cuh:283-407, the 10-point LB_Kim: an unconstrained DTW over the
+// front 5x5 corner (query rows q[0..5), stream columns z[0..5)) entered at
+// (0, 0), whose minimum over its exit boundary (last column, last row) starts
+// the bound, plus the back 5x5 corner entered anywhere on its top row or left
+// column, read at (4, 4).  Needs m >= 11 (bounds.cuh:418).  Cells are
+// cost + min(left, min(diagonal, up)), cost then min in that order, so the
+// rounding matches the CPU restatement; the tile is fully unrolled into
+// registers (five stream values, one row of cells).
+template <bool L2, bool EXACT, bool FRONT>
+__device__ __forceinline__ float kim_corner(const float* __restrict__ x,
+                                            const float* __restrict__ q, float mu, float sg,
+                                            float rs, float* exit_min) {
+    float z[5], row[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) z[c] = zn<EXACT>(x[c], mu, sg, rs);
+    float ex = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+        const float qr = q[r];
+        float left = 0.0f, diag = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const float up = row[c];  // row r - 1 (unused in row 0)
+            const float d = qr - z[c];
+            float v;
+            if (r == 0)
+                v = (FRONT && c > 0) ? lpa<L2>(d, left) : lp<L2>(d);
+            else if (c == 0)
+                v = FRONT ? lpa<L2>(d, up) : lp<L2>(d);
+            else
+                v = lpa<L2>(d, fminf(left, fminf(diag, up)));
+            diag = up;
+            left = v;
+            row[c] = v;
+        }
+        if constexpr (FRONT) {
+            if (r == 0)
+                ex = row[4];
+            else if (r < 4)
+                ex = fminf(ex, row[4]);
+            else
+                ex = fminf(ex, fminf(row[0], fminf(row[1], fminf(row[2], fminf(row[3], row[4])))));
+        }
+    }
+    if constexpr (FRONT) *exit_min = ex;
+    return row[4];
+}
+
 template <bool L2, bool EXACT>
 __device__ __forceinline__ float lb_kim_window(const float* __restrict__ x,
                                                const float* __restrict__ q, int32_t m,
                                                float mu, float sg, float rs) {
-    float q0, q1, q2, q3, q4, s0, s1, s2, s3, s4, mem;
-    float p0, p1, p2, p3, p4, p5, p6, p7, p8, p9;
-    q0 = q[0]; q1 = q[1]; q2 = q[2]; q3 = q[3]; q4 = q[4];
-    s0 = zn<EXACT>(x[0], mu, sg, rs);
-    s1 = zn<EXACT>(x[1], mu, sg, rs);
-    s2 = zn<EXACT>(x[2], mu, sg, rs);
-    s3 = zn<EXACT>(x[3], mu, sg, rs);
-    s4 = zn<EXACT>(x[4], mu, sg, rs);
-
-    p0 = lp<L2>(q0 - s0);
-    p1 = lpa<L2>(q0 - s1, p0);
-    p2 = lpa<L2>(q0 - s2, p1);
-    p3 = lpa<L2>(q0 - s3, p2);
-    p4 = lpa<L2>(q0 - s4, p3);
-    mem = p4;
-
-    p5 = lpa<L2>(q1 - s0, p0);
-    p6 = lpa<L2>(q1 - s1, fminf(p5, fminf(p0, p1)));
-    p7 = lpa<L2>(q1 - s2, fminf(p6, fminf(p1, p2)));
-    p8 = lpa<L2>(q1 - s3, fminf(p7, fminf(p2, p3)));
-    p9 = lpa<L2>(q1 - s4, fminf(p8, fminf(p3, p4)));
-    mem = fminf(mem, p9);
-
-    p0 = lpa<L2>(q2 - s0, p5);
-    p1 = lpa<L2>(q2 - s1, fminf(p0, fminf(p5, p6)));
-    p2 = lpa<L2>(q2 - s2, fminf(p1, fminf(p6, p7)));
-    p3 = lpa<L2>(q2 - s3, fminf(p2, fminf(p7, p8)));
-    p4 = lpa<L2>(q2 - s4, fminf(p3, fminf(p8, p9)));
-    mem = fminf(mem, p4);
-
-    p5 = lpa<L2>(q3 - s0, p0);
-    p6 = lpa<L2>(q3 - s1, fminf(p5, fminf(p0, p1)));
-    p7 = lpa<L2>(q3 - s2, fminf(p6, fminf(p1, p2)));
-    p8 = lpa<L2>(q3 - s3, fminf(p7, fminf(p2, p3)));
-    p9 = lpa<L2>(q3 - s4, fminf(p8, fminf(p3, p4)));
-    mem = fminf(mem, p9);
-
-    p0 = lpa<L2>(q4 - s0, p5);
-    p1 = lpa<L2>(q4 - s1, fminf(p0, fminf(p5, p6)));
-    p2 = lpa<L2>(q4 - s2, fminf(p1, fminf(p6, p7)));
-    p3 = lpa<L2>(q4 - s3, fminf(p2, fminf(p7, p8)));
-    p4 = lpa<L2>(q4 - s4, fminf(p3, fminf(p8, p9)));
-    mem = fminf(mem, fminf(p0, fminf(p1, fminf(p2, fminf(p3, p4)))));
-
-    q0 = q[m - 5]; q1 = q[m - 4]; q2 = q[m - 3]; q3 = q[m - 2]; q4 = q[m - 1];
-    s0 = zn<EXACT>(x[m - 5], mu, sg, rs);
-    s1 = zn<EXACT>(x[m - 4], mu, sg, rs);
-    s2 = zn<EXACT>(x[m - 3], mu, sg, rs);
-    s3 = zn<EXACT>(x[m - 2], mu, sg, rs);
-    s4 = zn<EXACT>(x[m - 1], mu, sg, rs);
-
-    p0 = lp<L2>(q0 - s0);
-    p1 = lp<L2>(q0 - s1);
-    p2 = lp<L2>(q0 - s2);
-    p3 = lp<L2>(q0 - s3);
-    p4 = lp<L2>(q0 - s4);
-
-    p5 = lp<L2>(q1 - s0);
-    p6 = lpa<L2>(q1 - s1, fminf(p5, fminf(p0, p1)));
-    p7 = lpa<L2>(q1 - s2, fminf(p6, fminf(p1, p2)));
-    p8 = lpa<L2>(q1 - s3, fminf(p7, fminf(p2, p3)));
-    p9 = lpa<L2>(q1 - s4, fminf(p8, fminf(p3, p4)));
-
-    p0 = lp<L2>(q2 - s0);
-    p1 = lpa<L2>(q2 - s1, fminf(p0, fminf(p5, p6)));
-    p2 = lpa<L2>(q2 - s2, fminf(p1, fminf(p6, p7)));
-    p3 = lpa<L2>(q2 - s3, fminf(p2, fminf(p7, p8)));
-    p4 = lpa<L2>(q2 - s4, fminf(p3, fminf(p8, p9)));
-
-    p5 = lp<L2>(q3 - s0);
-    p6 = lpa<L2>(q3 - s1, fminf(p5, fminf(p0, p1)));
-    p7 = lpa<L2>(q3 - s2, fminf(p6, fminf(p1, p2)));
-    p8 = lpa<L2>(q3 - s3, fminf(p7, fminf(p2, p3)));
-    p9 = lpa<L2>(q3 - s4, fminf(p8, fminf(p3, p4)));
-
-    p0 = lp<L2>(q4 - s0);
-    p1 = lpa<L2>(q4 - s1, fminf(p0, fminf(p5, p6)));
-    p2 = lpa<L2>(q4 - s2, fminf(p1, fminf(p6, p7)));
-    p3 = lpa<L2>(q4 - s3, fminf(p2, fminf(p7, p8)));
-    p4 = lpa<L2>(q4 - s4, fminf(p3, fminf(p8, p9)));
-
-    return mem + p4;
+    float front = 0.0f;
+    (void)kim_corner<L2, EXACT, true>(x, q, mu, sg, rs, &front);
+    const float back = kim_corner<L2, EXACT, false>(x + (m - 5), q + (m - 5), mu, sg, rs, nullptr);
+    return front + back;
 }
 
 template <bool L2>

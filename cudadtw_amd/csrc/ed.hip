@@ -89,15 +89,214 @@ __global__ __launch_bounds__(256) void ed_kernel(
     }
 }
 
+
+// Contiguous windows (stride 1, no index list), four per lane.  Every lane of
+// ed_kernel loads its own copy of each sample (64 lanes x 4 bytes per element,
+// 63 of them shared with the neighbours): the vector-memory path, not the
+// VALU, bounded it (4 VALU per element, ~40% of the VALU's issue rate).  Here
+// lane l of a wave owns windows w0 + 4l + j (j < 4): for query positions
+// k..k+3 its four windows need samples x[w0 + 4l + k .. + 7), two aligned
+// 16-byte vectors, of which the second is the next step's first -- one
+// 16-byte load per lane per four query positions serves sixteen
+// window-elements, and a wave's loads are one contiguous kilobyte.  The
+// arithmetic per element is ed_kernel's (sub, mul by the reciprocal, sub,
+// fma, in that order), so the distances are bit-identical.
+constexpr int kEd4 = 4;                      // windows per lane
+constexpr int kEd4Tile = kWave * kEd4;       // windows per wave
+
+__device__ __forceinline__ float4 ed4_load(const float* __restrict__ s, int64_t i) {
+    return *reinterpret_cast<const float4*>(s + i);
+}
+
+__device__ __forceinline__ float ed4_pick(const float4& a, const float4& b, int i) {
+    switch (i) {
+        case 0: return a.x;
+        case 1: return a.y;
+        case 2: return a.z;
+        case 3: return a.w;
+        case 4: return b.x;
+        case 5: return b.y;
+        case 6: return b.z;
+        default: return b.w;
+    }
+}
+
+// Four query positions k..k+3 of the lane's four windows: window j's samples
+// are x[w0 + j + k + i], i < 4, the elements i + j of the vector pair (a, b).
+__device__ __forceinline__ void ed4_chunk(const float4& a, const float4& b,
+                                          const float* __restrict__ qk, const float (&mu)[kEd4],
+                                          const float (&rs)[kEd4], float (&acc)[kEd4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float qi = qk[i];
+#pragma unroll
+        for (int j = 0; j < kEd4; ++j) {
+            const float z = (ed4_pick(a, b, i + j) - mu[j]) * rs[j];
+            const float d = qi - z;
+            acc[j] = fmaf(d, d, acc[j]);
+        }
+    }
+}
+
+__device__ __forceinline__ bool ed4_check(bool (&alive)[kEd4], const float (&acc)[kEd4],
+                                          float thr) {
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < kEd4; ++j) {
+        alive[j] = alive[j] && !(acc[j] > thr);
+        any = any || alive[j];
+    }
+    return __any(any);
+}
+
+__device__ __forceinline__ void ed4_windows(const float* __restrict__ s, int64_t w0,
+                                            const float* __restrict__ q,
+                                            int32_t m, const float (&mu)[kEd4],
+                                            const float (&rs)[kEd4], bool (&alive)[kEd4],
+                                            float thr, bool abandon, float (&acc)[kEd4]) {
+    // v(c) = x[w0 + 4c .. w0 + 4c + 4); chunk c needs v(c) and v(c + 1).
+    // Three chunks per iteration rotate three vectors through r0, r1, r2 so
+    // no vector is copied between registers; each load is issued a chunk
+    // before its first use.
+    auto v = [&](int32_t c) { return ed4_load(s, w0 + 4 * static_cast<int64_t>(c)); };
+    float4 r0 = v(0), r1 = v(1);
+    int32_t k = 0;
+    for (; k + 12 <= m; k += 12) {
+        const int32_t c = k / 4;
+        const float4 r2 = v(c + 2);
+        ed4_chunk(r0, r1, q + k, mu, rs, acc);
+        r0 = v(c + 3);
+        ed4_chunk(r1, r2, q + k + 4, mu, rs, acc);
+        r1 = v(c + 4);
+        ed4_chunk(r2, r0, q + k + 8, mu, rs, acc);
+        if (abandon && !ed4_check(alive, acc, thr)) return;
+    }
+    for (; k + 4 <= m; k += 4) {
+        const float4 r2 = v(k / 4 + 2);
+        ed4_chunk(r0, r1, q + k, mu, rs, acc);
+        r0 = r1;
+        r1 = r2;
+    }
+    // last m % 4 query positions (wave-uniform count)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        if (k + i >= m) break;
+        const float qi = q[k + i];
+#pragma unroll
+        for (int j = 0; j < kEd4; ++j) {
+            const float z = (ed4_pick(r0, r1, i + j) - mu[j]) * rs[j];
+            const float d = qi - z;
+            acc[j] = fmaf(d, d, acc[j]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void ed4_kernel(
+    const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
+    const float* __restrict__ q, float* __restrict__ out, unsigned long long* __restrict__ res,
+    Ctrl* __restrict__ ctrl, BandScalars a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+    const int32_t m = a.m;
+    // windows [first, end); the host launches only tiles whose 16-byte loads
+    // stay inside the stream (a.stride = their count), the rest go to ed_kernel
+    const int64_t first = a.base, end = a.base + a.count;
+    const int64_t tile0 = (first / kEd4Tile) * kEd4Tile;
+    const int64_t ntiles = a.stride;
+    const bool abandon = a.mode == kModeBest || a.mode == kModeCollect;
+    for (int64_t t = wave; t < ntiles; t += nwaves) {
+        const int64_t wt = tile0 + t * kEd4Tile;
+        const int64_t w0 = wt + kEd4 * lane;
+        float mu[kEd4], rs[kEd4], acc[kEd4];
+        bool valid[kEd4], alive[kEd4];
+#pragma unroll
+        for (int j = 0; j < kEd4; ++j) {
+            const int64_t w = w0 + j;
+            valid[j] = w >= first && w < end;
+            const int64_t wc = w < first ? first : (w < end ? w : end - 1);
+            mu[j] = avg[wc];
+            rs[j] = 1.0f / sd[wc];
+            alive[j] = valid[j];
+            acc[j] = 0.0f;
+        }
+        float thr = kInf;
+        if (abandon) thr = a.mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
+        ed4_windows(s, w0, q, m, mu, rs, alive, thr, abandon, acc);
+        unsigned long long best = ~0ull;
+        bool keep[kEd4];
+#pragma unroll
+        for (int j = 0; j < kEd4; ++j) {
+            const int64_t w = w0 + j;
+            const bool done = alive[j];  // not abandoned
+            const float d = done ? acc[j] : kInf;
+            if (a.mode == kModeProfile && valid[j]) out[w - first] = d;
+            if (a.mode == kModeBest && done) {
+                const unsigned long long key = pack_key(d, static_cast<uint32_t>(w));
+                best = key < best ? key : best;
+            }
+            keep[j] = a.mode == kModeCollect && done && d <= thr;
+            if (keep[j]) note_bucket(ctrl->bmin, ctrl->bucket_len, w, d);
+        }
+        if (a.mode == kModeBest) {
+            best = wave_min_u64(best);
+            if (lane == 0 && best != ~0ull) atomicMin(&ctrl->best, best);
+        } else if (a.mode == kModeCollect) {
+#pragma unroll
+            for (int j = 0; j < kEd4; ++j) {
+                const unsigned long long km = __ballot(keep[j]);
+                if (km) {
+                    unsigned long long rb = 0;
+                    if (lane == 0)
+                        rb = atomicAdd(&ctrl->nres, static_cast<unsigned long long>(__popcll(km)));
+                    rb = __shfl(rb, 0, kWave);
+                    if (keep[j])
+                        res[rb + ed_rank(km)] = pack_key(acc[j], static_cast<uint32_t>(w0 + j));
+                }
+            }
+        }
+    }
+}
+
 }  // namespace
 
 int launch_ed(hipStream_t st, int cus, const BandArgs& a) {
     if (a.count <= 0) return MDTW_OK;
     BandScalars sc{a.m, 0, a.count, a.base, a.stride, a.mode, 0};
-    const int64_t waves = (a.count + kWave - 1) / kWave;
-    const int64_t blocks = std::min<int64_t>((waves + 3) / 4, static_cast<int64_t>(cus) * 64);
-    hipLaunchKernelGGL(ed_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, a.s,
-                       a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc);
+    // Contiguous windows of a 16-byte aligned stream: four windows per lane
+    // for every tile of 256 whose vector loads (up to 12 samples past its
+    // last window's end) stay within the samples the windows read; ed_kernel
+    // takes the windows after those tiles.
+    const bool contiguous = !a.idx && a.stride == 1 && a.mode != kModeSeed &&
+                            (reinterpret_cast<uintptr_t>(a.s) & 15) == 0;
+    int64_t done = 0;  // windows [a.base, a.base + done) handled by ed4_kernel
+    if (contiguous) {
+        const int64_t first = a.base, end = a.base + a.count;
+        const int64_t samples_end = end + a.m - 1;  // one past the last sample read
+        const int64_t tile0 = (first / kEd4Tile) * kEd4Tile;
+        // tile t is fast when tile0 + (t + 1) * 256 + m + 12 <= samples_end
+        const int64_t room = samples_end - a.m - 12 - tile0;
+        const int64_t nfast = room >= kEd4Tile ? room / kEd4Tile : 0;
+        if (nfast > 0) {
+            BandScalars s4 = sc;
+            s4.stride = nfast;  // tile count, in the stride slot
+            const int64_t blocks = std::min<int64_t>((nfast + 3) / 4, static_cast<int64_t>(cus) * 64);
+            hipLaunchKernelGGL(ed4_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
+                               a.s, a.avg, a.sd, a.qx, a.out, a.res, a.ctrl, s4);
+            done = std::min<int64_t>(a.count, tile0 + nfast * kEd4Tile - first);
+        }
+    }
+    if (done < a.count) {
+        BandScalars rest = sc;
+        rest.count = a.count - done;
+        rest.base = a.base + done * a.stride;
+        float* out = a.out && done ? a.out + done : a.out;
+        const int32_t* idx = a.idx ? a.idx + done : nullptr;
+        const int64_t waves = (rest.count + kWave - 1) / kWave;
+        const int64_t blocks = std::min<int64_t>((waves + 3) / 4, static_cast<int64_t>(cus) * 64);
+        hipLaunchKernelGGL(ed_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, a.s,
+                           a.avg, a.sd, a.qx, idx, out, a.res, a.ctrl, rest);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

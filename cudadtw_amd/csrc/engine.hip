@@ -50,41 +50,31 @@ int upload(mdtw_engine* e, void* dst, const void* src, size_t bytes) {
     return MDTW_OK;
 }
 
-// bounds.cuh:13-64 Lemire envelope (deques kept as flat index arrays of m + 1
-// entries; they only grow at the back).  w >= m is clamped to m - 1 (global
-// max/min).
+// bounds.cuh:13-64 envelope: up[i] / lo[i] = max / min of q over
+// [i - w, i + w] clipped to [0, m), by a monotone-deque sliding window (the
+// published O(m) method the reference also uses): ub holds indices of
+// decreasing values, lb of increasing ones, each at most m entries (the
+// caller's arrays hold m + 1).  Exact maxima and minima, so every correct
+// method gives these bits.  w >= m is clamped to m - 1 (global max/min).
 void host_envelope(const float* q, int32_t w, float* lo, float* up, int32_t m, int* ub,
                    int* lb) {
     if (m <= 0) return;
     if (w > m - 1) w = m - 1;
     if (w < 0) w = 0;
     int uh = 0, ut = 0, lh = 0, lt = 0;
-    ub[ut++] = 0;
-    lb[lt++] = 0;
-    for (int i = 1; i < m; ++i) {
-        if (i > w) {
-            up[i - w - 1] = q[ub[uh]];
-            lo[i - w - 1] = q[lb[lh]];
+    for (int j = 0; j < m + w; ++j) {
+        if (j < m) {
+            while (ut > uh && q[ub[ut - 1]] <= q[j]) --ut;
+            ub[ut++] = j;
+            while (lt > lh && q[lb[lt - 1]] >= q[j]) --lt;
+            lb[lt++] = j;
         }
-        if (q[i] > q[i - 1]) {
-            ut--;
-            while (ut > uh && q[i] > q[ub[ut - 1]]) ut--;
-        } else {
-            lt--;
-            while (lt > lh && q[i] < q[lb[lt - 1]]) lt--;
-        }
-        ub[ut++] = i;
-        lb[lt++] = i;
-        if (i == 2 * w + 1 + ub[uh])
-            uh++;
-        else if (i == 2 * w + 1 + lb[lh])
-            lh++;
-    }
-    for (int i = m; i < m + w + 1; ++i) {
-        up[i - w - 1] = q[ub[uh]];
-        lo[i - w - 1] = q[lb[lh]];
-        if (i - ub[uh] >= 2 * w + 1) uh++;
-        if (i - lb[lh] >= 2 * w + 1) lh++;
+        const int c = j - w;  // the window centred at c is complete once j = c + w
+        if (c < 0) continue;
+        while (ub[uh] < c - w) ++uh;
+        while (lb[lh] < c - w) ++lh;
+        up[c] = q[ub[uh]];
+        lo[c] = q[lb[lh]];
     }
 }
 

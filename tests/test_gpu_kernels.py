@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import torch
 
-from cudadtw_amd import METRIC_L1, METRIC_L2, STATS_FP64, STATS_REFCOMPAT, _lib, synth
+from cudadtw_amd import ALGO_ED, METRIC_L1, METRIC_L2, STATS_FP64, STATS_REFCOMPAT, _lib, synth
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -258,6 +258,35 @@ def test_ed_vs_fp64(engine, m):
     ref = O.ed_f64(zq.astype(np.float64), s, a64, s64)
     assert np.max(np.abs(got - ref) / np.maximum(ref, 1.0)) < 1e-5
     assert int(np.argmin(got)) == int(np.argmin(ref))
+
+
+@pytest.mark.parametrize("n,m", [(300, 256), (1000, 256), (1 << 15, 16), (40_000, 1000),
+                                 (7777, 333), (70_000, 2048)])
+def test_ed_four_window_kernel_equals_per_lane_kernel(engine, n, m):
+    # A 16-byte aligned stream takes the four-windows-per-lane kernel for its
+    # interior tiles (and the per-lane kernel for the last windows); the same
+    # samples one float off alignment take the per-lane kernel throughout.
+    # Both do the same arithmetic per element: bit-identical profiles.
+    s = synth.ecg_like(n, m)
+    q = synth.ecg_query(m, 2) if m >= 200 else synth.random_walk(m, 4)
+    zq = O.znormalize_f32(q)
+    engine.set_query(zq, 0)
+    ds = dev(s)
+    padded = dev(np.concatenate([[0.0], s]).astype(np.float32))
+    ds_off = padded[1:]
+    assert ds.data_ptr() % 16 == 0 and ds_off.data_ptr() % 16 == 4
+    avg, sd = engine.avg_std(ds, m)
+    a = host(engine.ed(ds, avg, sd))
+    b = host(engine.ed(ds_off, avg, sd))
+    assert np.array_equal(a, b)
+    _, _, a64, s64 = O.stats_fp64(s, m, want64=True)
+    ref = O.ed_f64(zq.astype(np.float64), s, a64, s64)
+    assert np.max(np.abs(a - ref) / np.maximum(ref, 1.0)) < 1e-5
+    for k in (1, 4):  # searches through both kernels agree too
+        r1 = engine.search(ds, 0, algo=ALGO_ED, k=k, excl=m // 2)
+        r2 = engine.search(ds_off, 0, algo=ALGO_ED, k=k, excl=m // 2)
+        assert np.array_equal(r1.positions, r2.positions)
+        assert np.array_equal(r1.distances, r2.distances)
 
 
 # ---- primitives (tests/test_threshold.cu) -------------------------------------------------------
