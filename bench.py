@@ -255,6 +255,34 @@ def section_c2(eng, torch, args):
         gbps = L * nbytes / (ms / 1e3) / 1e9
         mem[name] = {"ms": round(ms, 3), "gb_per_s": round(gbps, 1),
                      "frac_of_8tbs": round(gbps / 8000.0, 3), "bytes_per_window": nbytes}
+    # the ED profile against its own roof: 2M + 6 flop per window (one
+    # multiply-add per element of the correlation form plus the window's
+    # normalisation), and the VALU issue it needs (4 VOP2 per element: sub, mul, sub,
+    # fma, each 2 cycles per wave64 with packed FP32 off) against the FP32
+    # VOP2 issue rate of the chip (256 CU x 4 SIMD x 32 lanes x 2.4 GHz)
+    ed_ms = mem["ed_profile"]["ms"]
+    ed_tflops = L * (2 * m + 6) / (ed_ms / 1e3) / 1e12
+    issue = L * m * 4 / (ed_ms / 1e3) / (256 * 4 * 32 * 2.4e9)
+    mem["ed_profile"]["roofline"] = {"bound": "valu", "achieved": round(ed_tflops, 2),
+                                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": round(ed_tflops / PEAK_FP32_TFLOPS, 4),
+                                     "flops_per_window": 2 * m + 6,
+                                     "valu_issue_frac": round(issue, 3)}
+    # a plain device copy of the stream (torch's copy kernel): the HBM rate a
+    # 1 read : 1 write pattern reaches here, beside the kernels above
+    dst = torch.empty_like(ds)
+    dst.copy_(ds)
+    st.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(3):
+        dst.copy_(ds)
+    e1.record(st)
+    torch.cuda.synchronize()
+    cms = e0.elapsed_time(e1) / 3
+    mem["device_copy"] = {"ms": round(cms, 3), "bytes": 2 * 4 * n,
+                          "gb_per_s": round(2 * 4 * n / (cms / 1e3) / 1e9, 1)}
+    del dst
     log(f"[c2] memory-side kernels over all windows: {mem}")
 
     # DP rate on wide bands (|Q| = 1024 at 10 / 15.6 / 20 %: W = 102 in the

@@ -138,22 +138,10 @@ __device__ __forceinline__ void ed4_chunk(const float4& a, const float4& b,
     }
 }
 
-__device__ __forceinline__ bool ed4_check(bool (&alive)[kEd4], const float (&acc)[kEd4],
-                                          float thr) {
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < kEd4; ++j) {
-        alive[j] = alive[j] && !(acc[j] > thr);
-        any = any || alive[j];
-    }
-    return __any(any);
-}
-
 __device__ __forceinline__ void ed4_windows(const float* __restrict__ s, int64_t w0,
                                             const float* __restrict__ q,
                                             int32_t m, const float (&mu)[kEd4],
-                                            const float (&rs)[kEd4], bool (&alive)[kEd4],
-                                            float thr, bool abandon, float (&acc)[kEd4]) {
+                                            const float (&rs)[kEd4], float (&acc)[kEd4]) {
     // v(c) = x[w0 + 4c .. w0 + 4c + 4); chunk c needs v(c) and v(c + 1).
     // Three chunks per iteration rotate three vectors through r0, r1, r2 so
     // no vector is copied between registers; each load is issued a chunk
@@ -169,7 +157,6 @@ __device__ __forceinline__ void ed4_windows(const float* __restrict__ s, int64_t
         ed4_chunk(r1, r2, q + k + 4, mu, rs, acc);
         r1 = v(c + 4);
         ed4_chunk(r2, r0, q + k + 8, mu, rs, acc);
-        if (abandon && !ed4_check(alive, acc, thr)) return;
     }
     for (; k + 4 <= m; k += 4) {
         const float4 r2 = v(k / 4 + 2);
@@ -191,10 +178,10 @@ __device__ __forceinline__ void ed4_windows(const float* __restrict__ s, int64_t
     }
 }
 
+// Profile only (every window's distance, no abandoning).
 __global__ __launch_bounds__(256) void ed4_kernel(
     const float* __restrict__ s, const float* __restrict__ avg, const float* __restrict__ sd,
-    const float* __restrict__ q, float* __restrict__ out, unsigned long long* __restrict__ res,
-    Ctrl* __restrict__ ctrl, BandScalars a) {
+    const float* __restrict__ q, float* __restrict__ out, BandScalars a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wave = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
@@ -204,56 +191,22 @@ __global__ __launch_bounds__(256) void ed4_kernel(
     const int64_t first = a.base, end = a.base + a.count;
     const int64_t tile0 = (first / kEd4Tile) * kEd4Tile;
     const int64_t ntiles = a.stride;
-    const bool abandon = a.mode == kModeBest || a.mode == kModeCollect;
     for (int64_t t = wave; t < ntiles; t += nwaves) {
-        const int64_t wt = tile0 + t * kEd4Tile;
-        const int64_t w0 = wt + kEd4 * lane;
+        const int64_t w0 = tile0 + t * kEd4Tile + kEd4 * lane;
         float mu[kEd4], rs[kEd4], acc[kEd4];
-        bool valid[kEd4], alive[kEd4];
 #pragma unroll
         for (int j = 0; j < kEd4; ++j) {
             const int64_t w = w0 + j;
-            valid[j] = w >= first && w < end;
             const int64_t wc = w < first ? first : (w < end ? w : end - 1);
             mu[j] = avg[wc];
             rs[j] = 1.0f / sd[wc];
-            alive[j] = valid[j];
             acc[j] = 0.0f;
         }
-        float thr = kInf;
-        if (abandon) thr = a.mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
-        ed4_windows(s, w0, q, m, mu, rs, alive, thr, abandon, acc);
-        unsigned long long best = ~0ull;
-        bool keep[kEd4];
+        ed4_windows(s, w0, q, m, mu, rs, acc);
 #pragma unroll
         for (int j = 0; j < kEd4; ++j) {
             const int64_t w = w0 + j;
-            const bool done = alive[j];  // not abandoned
-            const float d = done ? acc[j] : kInf;
-            if (a.mode == kModeProfile && valid[j]) out[w - first] = d;
-            if (a.mode == kModeBest && done) {
-                const unsigned long long key = pack_key(d, static_cast<uint32_t>(w));
-                best = key < best ? key : best;
-            }
-            keep[j] = a.mode == kModeCollect && done && d <= thr;
-            if (keep[j]) note_bucket(ctrl->bmin, ctrl->bucket_len, w, d);
-        }
-        if (a.mode == kModeBest) {
-            best = wave_min_u64(best);
-            if (lane == 0 && best != ~0ull) atomicMin(&ctrl->best, best);
-        } else if (a.mode == kModeCollect) {
-#pragma unroll
-            for (int j = 0; j < kEd4; ++j) {
-                const unsigned long long km = __ballot(keep[j]);
-                if (km) {
-                    unsigned long long rb = 0;
-                    if (lane == 0)
-                        rb = atomicAdd(&ctrl->nres, static_cast<unsigned long long>(__popcll(km)));
-                    rb = __shfl(rb, 0, kWave);
-                    if (keep[j])
-                        res[rb + ed_rank(km)] = pack_key(acc[j], static_cast<uint32_t>(w0 + j));
-                }
-            }
+            if (w >= first && w < end) out[w - first] = acc[j];
         }
     }
 }
@@ -263,11 +216,15 @@ __global__ __launch_bounds__(256) void ed4_kernel(
 int launch_ed(hipStream_t st, int cus, const BandArgs& a) {
     if (a.count <= 0) return MDTW_OK;
     BandScalars sc{a.m, 0, a.count, a.base, a.stride, a.mode, 0};
-    // Contiguous windows of a 16-byte aligned stream: four windows per lane
-    // for every tile of 256 whose vector loads (up to 12 samples past its
-    // last window's end) stay within the samples the windows read; ed_kernel
-    // takes the windows after those tiles.
-    const bool contiguous = !a.idx && a.stride == 1 && a.mode != kModeSeed &&
+    // Profile of contiguous windows of a 16-byte aligned stream: four windows
+    // per lane for every tile of 256 whose vector loads (up to 12 samples
+    // past its last window's end) stay within the samples the windows read;
+    // ed_kernel takes the windows after those tiles.  The abandoning modes
+    // stay on ed_kernel: a wave abandons only when all of its windows are
+    // above the threshold, and 256 neighbouring windows reach that point
+    // later than 64 do (at |Q| = 8192 the four-window form took 2.4x longer
+    // per search, at |Q| = 256 about 5% less).
+    const bool contiguous = !a.idx && a.stride == 1 && a.mode == kModeProfile &&
                             (reinterpret_cast<uintptr_t>(a.s) & 15) == 0;
     int64_t done = 0;  // windows [a.base, a.base + done) handled by ed4_kernel
     if (contiguous) {
@@ -282,7 +239,7 @@ int launch_ed(hipStream_t st, int cus, const BandArgs& a) {
             s4.stride = nfast;  // tile count, in the stride slot
             const int64_t blocks = std::min<int64_t>((nfast + 3) / 4, static_cast<int64_t>(cus) * 64);
             hipLaunchKernelGGL(ed4_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st,
-                               a.s, a.avg, a.sd, a.qx, a.out, a.res, a.ctrl, s4);
+                               a.s, a.avg, a.sd, a.qx, a.out, s4);
             done = std::min<int64_t>(a.count, tile0 + nfast * kEd4Tile - first);
         }
     }
