@@ -42,41 +42,28 @@ constexpr int kBandWaves = 4;  // waves per workgroup
 constexpr int kBandSlideMax = 54;
 // Bands up to kBandMaxW unroll kBandU columns per block (W = 108..110: two,
 // W = 111..112: one, which keeps them within 256 registers, i.e. two waves
-// per SIMD); wider bands (up to kBandWideMaxW) keep the 2W + 1 band values
-// in vector registers (two waves per SIMD up to W = 114, one column at a
-// time), in vector registers plus an LDS tail (two waves per SIMD, below), or
-// in vector + accumulation registers at one wave per SIMD, four or two columns at a
-// time, skewed (band_skewed).
+// per SIMD); W = 113..114 keep the 2W + 1 band values in vector registers,
+// one column at a time (two waves per SIMD), W = 115..kBandTailMaxW in vector
+// registers plus an LDS tail (below), and the bands from kBandPairMinW to
+// kBandWideMaxW are split over two waves (cdtw_pair.hpp).
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
-// Bands W = 115..kBandTailMaxW (L2; L1 to kBandTailMaxWL1) keep band values
-// 0..kBandTailR-1 in vector registers and the rest (the tail, t >= kBandTailR)
-// in LDS, 16 bytes per lane per chunk of four values: at most 16 chunks,
-// 64 KiB per workgroup, so two workgroups (two waves per SIMD, the register
-// budget of 256) share a CU's 160 KiB.  Without the tail these bands need
-// accumulation registers and run at one wave per SIMD, where each column's
-// serial min3 -> fmac chain is exposed.  Their kernels are compiled for two
-// waves per SIMD (band_kernel_tail); the compiler then spills a few band
-// values per column to scratch memory, a number that grows with the tail:
-// measured at M = 1024, the L2 tail kernel runs W = 115 at 18 T cells/s (one
-// wave per SIMD, one column at a time: 10), 120 at 16, 128 at 12 and 136 at
-// 10, while the one-wave skewed columns below run 128..136 at 11; the L1 tail
-// kernel falls to 9 at W = 130 and 7 at 134 (skewed columns: 11), hence the
-// two limits.
+// Bands W = 115..kBandTailMaxW keep band values 0..kBandTailR-1 in vector
+// registers and the rest (the tail, t >= kBandTailR) in LDS, 16 bytes per
+// lane per chunk of four values, so two workgroups (two waves per SIMD, the
+// register budget of 256) share a CU's 160 KiB.  Their kernels are compiled
+// for two waves per SIMD (band_kernel_tail); the compiler then spills a few
+// band values per column to scratch memory, a number that grows with the
+// tail: measured at M = 1024 (2^22 windows), L2, the tail kernel runs W = 115
+// at 17.7 T cells/s, 121 at 16.8, 124 at 12.9 and 136 at 10.7, where the
+// two-wave split runs 16.5..16.9 over the whole range; L1 alike (18.0 and
+// 17.2 at W = 115 and 120, 14.0 at 125, the split 15.3, 16.9, 16.9).
 constexpr int kBandTailR = 212;
-constexpr int kBandTailMaxW = 136;
-constexpr int kBandTailMaxWL1 = 129;
-__host__ __device__ constexpr int band_tail_len(int w, bool l2) {
-    return (w > 114 && w <= (l2 ? kBandTailMaxW : kBandTailMaxWL1)) ? 2 * w + 1 - kBandTailR : 0;
+constexpr int kBandTailMaxW = 121;
+__host__ __device__ constexpr int band_tail_len(int w) {
+    return (w > 114 && w <= kBandTailMaxW) ? 2 * w + 1 - kBandTailR : 0;
 }
-// Bands beyond the tail take their columns U at a time, skewed (band_dp):
-// four up to kBandSkew4MaxW, two beyond, where four made the compiler spill
-// the band to scratch memory.
-constexpr int kBandSkew4MaxW = 232;
-template <int W, bool L2>
-constexpr bool band_skewed() {
-    return band_tail_len(W, L2) == 0 && W > 114;
-}
+constexpr int kBandPairMinW = kBandTailMaxW + 1;
 constexpr int kBandTailStride = 4 * 64;  // float4 slots between a lane's chunks
 
 template <int NCHUNK>
@@ -122,17 +109,6 @@ __device__ __forceinline__ int64_t band_next(Ctrl* ctrl, int lane, int64_t first
     return static_cast<int64_t>((static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
-// d * d + mn with one rounding, as fmaf, but always the three-address
-// v_fma_f32: in the skewed pairs (two columns) the compiler's choice,
-// v_fmac_f32 (whose destination is its addend), left it spilling 100-230
-// bytes per lane to scratch memory at W = 216..236 (L2), where this form
-// spills none to 64.  (With four columns it is the other way round.)
-__device__ __forceinline__ float band_fma_sq(float d, float mn) {
-    float e;
-    asm("v_fma_f32 %0, %1, %1, %2" : "=v"(e) : "v"(d), "v"(mn));
-    return e;
-}
-
 // One column of the band update with the query values of this column in
 // q[0..B) (wave-uniform scalars).
 template <int B, bool L2, class Q>
@@ -159,14 +135,9 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                                          int mode, const Ctrl* ctrl,
                                          const float* __restrict__ elo,
                                          const float* __restrict__ eup, float lbt) {
+    static_assert(W < kBandPairMinW, "wider bands run split over two waves (cdtw_pair.hpp)");
     constexpr int B = 2 * W + 1;
-    // Bands wider than the LDS tail's range run one wave per SIMD and take
-    // their columns in skewed groups (kSkew).
-    constexpr bool kSkew = band_skewed<W, L2>();
-    constexpr int U =
-        kSkew ? (W <= kBandSkew4MaxW ? 4 : 2) : (W > 110 ? 1 : (W > 107 ? 2 : kBandU));
-    // L2 cell of two skewed columns: three-address fma (see band_fma_sq)
-    constexpr bool kFma3 = kSkew && U == 2;
+    constexpr int U = W > 110 ? 1 : (W > 107 ? 2 : kBandU);
     constexpr int CH = kBandChunk;
     constexpr int NC = (B + CH - 1) / CH;
     // Narrow bands keep the query window of a whole block of U columns in
@@ -177,7 +148,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     // query in chunks of CH scalars per column.
     constexpr bool kSlide = B + 2 * U - 1 <= kBandSlideMax;
     // KT band values in the LDS tail (0 for most bands), BR in registers.
-    constexpr int KT = band_tail_len(W, L2);
+    constexpr int KT = band_tail_len(W);
     constexpr int KT4 = (KT + 3) / 4;
     constexpr int BR = B - KT;
     static_assert(KT == 0 || (U == 1 && !kSlide && W < BR), "tail layout");
@@ -189,11 +160,6 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     // minimum of the tail's values after the last column (for the abandon
     // check, which then needs no LDS reads)
     [[maybe_unused]] float tail_min = kInf;
-    // skewed columns: E[W] of each but the last column of a block (the
-    // answer when M is not a multiple of U)
-    [[maybe_unused]] float ew[U > 1 ? U - 1 : 1];
-#pragma unroll
-    for (int u = 0; u + 1 < U; ++u) ew[u] = kInf;
     const float4 inf4 = make_float4(kInf, kInf, kInf, kInf);
     if constexpr (KT4 > 0) {
         tl = band_tail_lds<KT4>();
@@ -220,10 +186,10 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     const int lane = threadIdx.x & (kWave - 1);
     // Bands that fit in vector registers read the abandon bound once per
-    // batch of windows (load_abandon_bound); the wide ones re-read it at each
-    // check: their band already fills the accumulation registers, one more
-    // value live across the DP made the compiler spill it to scratch memory,
-    // and their checks are hundreds of cells apart anyway.
+    // batch of windows (load_abandon_bound); the wider ones re-read it at each
+    // check: their band already fills the 256 registers, one more value live
+    // across the DP made the compiler spill it to scratch memory, and their
+    // checks are hundreds of cells apart anyway.
     constexpr bool kBoundOnce = W <= kBandMaxW;
     const float thr_once = kBoundOnce && abandon ? load_abandon_bound(ctrl, mode) : kInf;
     auto bound = [&]() {
@@ -341,69 +307,6 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
 #pragma unroll
                 for (int i = 0; i < CH; ++i) qa[i] = qb[i];
             }
-        } else if constexpr (kSkew) {
-            // U columns per step, each two cells behind the one before.  Cell
-            // t of column j0+u needs cells t and t+1 of column j0+u-1,
-            // written one and two steps earlier, so the U dependency chains
-            // of a step are independent: the instruction-level parallelism a
-            // band this wide (one wave per SIMD) lacks with one column.  All
-            // update E in place: column j0+u overwrites cell s-2u at step s,
-            // reading the query u rows behind column j0 (qc[s-u]).  When M is
-            // not a multiple of U the last block's trailing columns are
-            // dummies (their sample index clamped): the answer is then column
-            // M-1's E[W], kept in ew, and that block skips the abandon check.
-            // (A separate path for a partial last block doubled the loop's
-            // code and sent the band to scratch memory.)
-            const float* __restrict__ qc = qx + j0;
-            constexpr int NS = (B + 2 * (U - 1) + CH - 1) / CH;
-            float p[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) p[u] = kInf;
-            // ql[k] = qc[ch * CH - 1 - k]: the previous chunk's last values
-            float ql[U - 1];
-#pragma unroll
-            for (int k = 0; k + 1 < U; ++k) ql[k] = kInf;
-            float qa[CH], qb[CH];
-            auto step = [&](int s, int i) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int t = s - 2 * u;
-                    if (t >= 0 && t < B) {
-                        const float nxt = t + 1 < B ? E[t + 1] : kInf;
-                        const float mn = fminf(fminf(p[u], E[t]), nxt);
-                        const float d = (i >= u ? qa[i - u] : ql[u - i - 1]) - z[u];
-                        const float e = L2 ? (kFma3 ? band_fma_sq(d, mn) : fmaf(d, d, mn))
-                                           : fabsf(d) + mn;
-                        E[t] = e;
-                        p[u] = e;
-                        if (u + 1 < U && t == W) ew[u] = e;
-                    }
-                }
-            };
-#pragma unroll
-            for (int i = 0; i < CH; ++i) qa[i] = qc[i];
-            // Scalar loads complete out of order, so every wait is for all of
-            // them: the next chunk is requested after the first step of this
-            // one (whose wait is for this chunk only) and a chunk ahead of its
-            // use.  The barriers keep that order (left free, the scheduler
-            // sinks the loads to just before their use, and one wave per SIMD
-            // has nothing to hide the wait).
-#pragma unroll
-            for (int ch = 0; ch < NS; ++ch) {
-                step(ch * CH, 0);
-                __builtin_amdgcn_sched_barrier(0);
-                if (ch + 1 < NS) {
-#pragma unroll
-                    for (int i = 0; i < CH; ++i) qb[i] = qc[(ch + 1) * CH + i];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int i = 1; i < CH; ++i) step(ch * CH + i, i);
-#pragma unroll
-                for (int k = 0; k + 1 < U; ++k) ql[k] = qa[CH - 1 - k];
-#pragma unroll
-                for (int i = 0; i < CH; ++i) qa[i] = qb[i];
-            }
         } else {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -442,8 +345,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
                 }
             }
         }
-        if (abandon && ((j0 / U) % (kBandCheck / U)) == (kBandCheck / U) - 1 &&
-            (!kSkew || j0 + U <= m)) {
+        if (abandon && ((j0 / U) % (kBandCheck / U)) == (kBandCheck / U) - 1) {
             float cm = E[0];
 #pragma unroll
             for (int t = 1; t < BR; ++t) cm = fminf(cm, E[t]);
@@ -460,14 +362,6 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
             }
             if (!__any(alive)) return kInf;
         }
-    }
-    if constexpr (kSkew) {
-        const int last = (m - 1) % U;  // column M-1's place in its block
-        float r = E[W];
-#pragma unroll
-        for (int u = 0; u + 1 < U; ++u)
-            if (last == u) r = ew[u];
-        return r;
     }
     return E[W];
 }
@@ -582,11 +476,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 template <int W, bool L2, bool LB>
 constexpr auto band_kernel_fn() {
-    if constexpr (band_tail_len(W, L2) > 0)
+    if constexpr (band_tail_len(W) > 0)
         return &band_kernel_tail<W, L2, LB>;
     else
         return &band_kernel<W, L2, LB>;
 }
+
+}  // namespace mdtw
+
+#include "cdtw_pair.hpp"
+
+namespace mdtw {
 
 // Waves per SIMD a kernel can keep resident, from its register count
 // (512 registers per lane per SIMD, allocated in granules of 8).
@@ -627,20 +527,34 @@ bool band_lb_pays() {
     return pays;
 }
 
+// `grid` counts workgroups of kBandWaves one-lane waves (256 windows per
+// pass); a split band takes four pairs of waves for each (the same windows per
+// pass, twice the waves: two or three of them fit a SIMD, against one or two
+// one-lane waves of these widths).
 template <int W, bool L2>
 int band_launch(hipStream_t st, unsigned grid, const BandArgs& a) {
     BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent,
                    0, a.qx_len};
-    const bool lb = a.lbt && a.idx && a.lo && a.up &&
-                    (a.mode == kModeBest || a.mode == kModeCollect) && band_lb_pays<W, L2>();
-    if (lb)
-        hipLaunchKernelGGL((band_kernel_fn<W, L2, true>()), dim3(grid), dim3(kBandWaves * kWave),
-                           0, st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, a.lbt,
-                           a.lo, a.up);
-    else
-        hipLaunchKernelGGL((band_kernel_fn<W, L2, false>()), dim3(grid), dim3(kBandWaves * kWave),
-                           0, st, a.s, a.avg, a.sd, a.qx, a.idx, a.out, a.res, a.ctrl, sc, nullptr,
-                           nullptr, nullptr);
+    if constexpr (W >= kBandPairMinW) {
+        const dim3 g(grid * kBandWaves), b(kPairWaves * kWave);
+        if constexpr (band_pair_tight(W))
+            hipLaunchKernelGGL((band_pair_kernel2<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
+                               a.idx, a.out, a.res, a.ctrl, sc);
+        else
+            hipLaunchKernelGGL((band_pair_kernel<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
+                               a.idx, a.out, a.res, a.ctrl, sc);
+    } else {
+        const bool lb = a.lbt && a.idx && a.lo && a.up &&
+                        (a.mode == kModeBest || a.mode == kModeCollect) && band_lb_pays<W, L2>();
+        if (lb)
+            hipLaunchKernelGGL((band_kernel_fn<W, L2, true>()), dim3(grid),
+                               dim3(kBandWaves * kWave), 0, st, a.s, a.avg, a.sd, a.qx, a.idx,
+                               a.out, a.res, a.ctrl, sc, a.lbt, a.lo, a.up);
+        else
+            hipLaunchKernelGGL((band_kernel_fn<W, L2, false>()), dim3(grid),
+                               dim3(kBandWaves * kWave), 0, st, a.s, a.avg, a.sd, a.qx, a.idx,
+                               a.out, a.res, a.ctrl, sc, nullptr, nullptr, nullptr);
+    }
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
         set_last_hip_error(err);

@@ -1,6 +1,7 @@
-// cdtw_band_4.hip -- exact-W instantiations of the register-band cDTW kernel
-// (cdtw_band.hpp), L2 metric, W in [113, 144] (one-column unroll); split
-// across units so they compile in parallel.
+// cdtw_band_4.hip -- exact-W instantiations of the banded cDTW kernels
+// (cdtw_band.hpp, cdtw_pair.hpp), L2 metric, W in [113, 144] (one column at a
+// time to W = 121, split over two waves beyond); split across units so they
+// compile in parallel.
 #include "cdtw_band.hpp"
 
 namespace mdtw {

@@ -1,6 +1,6 @@
-// cdtw_band_6.hip -- exact-W instantiations of the register-band cDTW kernel
-// (cdtw_band.hpp), L2 metric, W in [177, 208] (one-column unroll); split
-// across units so they compile in parallel.
+// cdtw_band_6.hip -- exact-W instantiations of the banded cDTW kernels
+// (cdtw_band.hpp, cdtw_pair.hpp), L2 metric, W in [177, 208] (split over two
+// waves); split across units so they compile in parallel.
 #include "cdtw_band.hpp"
 
 namespace mdtw {

@@ -1,7 +1,8 @@
-// cdtw_band_8.hip -- exact-W instantiations of the register-band cDTW kernel
-// (cdtw_band.hpp), L1 metric, W in [113, 144] (one-column unroll; the L2
-// instantiations of the same range are in cdtw_band_4.hip); split across
-// units so they compile in parallel.
+// cdtw_band_8.hip -- exact-W instantiations of the banded cDTW kernels
+// (cdtw_band.hpp, cdtw_pair.hpp), L1 metric, W in [113, 144] (one column at a
+// time to W = 121, split over two waves beyond; the L2 instantiations of the
+// same range are in cdtw_band_4.hip); split across units so they compile in
+// parallel.
 #include "cdtw_band.hpp"
 
 namespace mdtw {

@@ -1,7 +1,7 @@
-// cdtw_band_9.hip -- exact-W instantiations of the register-band cDTW kernel
-// (cdtw_band.hpp), L1 metric, W in [145, 176] (one-column unroll; the L2
-// instantiations of the same range are in cdtw_band_5.hip); split across
-// units so they compile in parallel.
+// cdtw_band_9.hip -- exact-W instantiations of the banded cDTW kernels
+// (cdtw_band.hpp, cdtw_pair.hpp), L1 metric, W in [145, 176] (split over two
+// waves; the L2 instantiations of the same range are in cdtw_band_5.hip); split
+// across units so they compile in parallel.
 #include "cdtw_band.hpp"
 
 namespace mdtw {

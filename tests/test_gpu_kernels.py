@@ -143,7 +143,8 @@ def test_stats_refcompat_matches_restatement(engine):
                                  (400, 137), (300, 144), (400, 145), (400, 177), (500, 208),
                                  (500, 209), (600, 240), (700, 241), (401, 137), (333, 150),
                                  (501, 201), (601, 240), (151, 160), (3, 150), (333, 215),
-                                 (334, 216), (335, 232), (337, 233), (3, 233), (2, 200)])
+                                 (334, 216), (335, 232), (337, 233), (3, 233), (2, 200),
+                                 (300, 121), (300, 122), (4, 130), (500, 225), (500, 226)])
 def test_cdtw_bit_exact_random_order(engine, m, w):
     n = 6000
     s = synth.random_walk(n, m + w)
@@ -154,6 +155,22 @@ def test_cdtw_bit_exact_random_order(engine, m, w):
     got = host(engine.gpu_cdtw(ds, avg, sd, w, indices=dev(idx, torch.int32)))
     ref = O.cdtw(zq, s, ha, hs, w, idx=idx, fused=1)
     assert np.array_equal(got, ref), np.max(np.abs(got - ref))
+
+
+@pytest.mark.parametrize("metric,w,m", [(METRIC_L2, 160, 300), (METRIC_L1, 205, 250),
+                                        (METRIC_L2, 240, 260)])
+def test_cdtw_split_band_many_passes(engine, metric, w, m):
+    # every window of a 2^20-sample stream (several grid passes of the split
+    # kernel), checked against the restated DP on 600 sampled windows
+    n = 1 << 20
+    s = synth.ecg_like(n, w + 5)
+    q, _ = synth.cut_query(s, m, 3)
+    zq, ds, avg, sd, ha, hs = prepare(engine, s, q, w)
+    got = host(engine.gpu_cdtw(ds, avg, sd, w, metric=metric))
+    L = n - m + 1
+    pick = np.sort(np.random.default_rng(5).choice(L, 600, replace=False)).astype(np.int32)
+    ref = O.cdtw(zq, s, ha, hs, w, idx=pick, metric=int(metric), fused=1)
+    assert np.array_equal(got[pick], ref), np.max(np.abs(got[pick] - ref))
 
 
 @pytest.mark.parametrize("w", [0, 13, 62, 63, 64, 100, 113, 127, 128, 200, 300])
@@ -172,11 +189,12 @@ def test_cdtw_wide_kernel_bit_exact(engine, w):
 
 @pytest.mark.parametrize("m,w", [(300, 112), (300, 113), (300, 115), (400, 136), (400, 137),
                                  (500, 200), (600, 240), (700, 241), (401, 151), (601, 239), (300, 129),
-                                 (300, 130), (401, 131), (402, 140), (403, 200)])
+                                 (300, 130), (401, 131), (402, 140), (403, 200), (300, 121),
+                                 (300, 122), (500, 226)])
 def test_cdtw_l1_wide_bands_bit_exact(engine, m, w):
-    # L1 bands above 112: register band (LDS tail at 115..129, skewed column
-    # pairs from 130) to W = 240,
-    # the anti-diagonal kernel beyond; both against the restated DP.
+    # L1 bands above 112: one lane per window to W = 121 (LDS tail from 115),
+    # the band split over two waves from 122 to 240, the anti-diagonal kernel
+    # beyond; both against the restated DP.
     n = 4000
     s = synth.random_walk(n, m + w + 7)
     zq, ds, avg, sd, ha, hs = prepare(engine, s, synth.random_walk(m, 98), w)

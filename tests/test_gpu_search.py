@@ -81,7 +81,7 @@ def test_refcompat_search_matches_restated_reference(engine):
     assert abs(r.distance / np.sqrt(prof.min()) - 1) < 1e-5
 
 
-@pytest.mark.parametrize("w", [13, 51, 120, 136, 150])
+@pytest.mark.parametrize("w", [13, 51, 120, 122, 136, 150, 200, 230, 240])
 def test_search_with_given_stats_and_wide_band(engine, w):
     s = synth.ecg_like(1 << 17, 21)
     q, pos = synth.cut_query(s, 512, 22)
