@@ -257,12 +257,12 @@ def section_c2(eng, torch, args):
                      "frac_of_8tbs": round(gbps / 8000.0, 3), "bytes_per_window": nbytes}
     # the ED profile against its own roof: 2M + 6 flop per window (one
     # multiply-add per element of the correlation form plus the window's
-    # normalisation), and the VALU issue it needs (4 VOP2 per element: sub, mul, sub,
-    # fma, each 2 cycles per wave64 with packed FP32 off) against the FP32
-    # VOP2 issue rate of the chip (256 CU x 4 SIMD x 32 lanes x 2.4 GHz)
+    # normalisation), and the VALU issue its element sequence needs (sub, fma,
+    # fma: 8.0 SIMD cycles per wave64 element, timed on this device) against
+    # the chip's 256 CU x 4 SIMD at 2.4 GHz
     ed_ms = mem["ed_profile"]["ms"]
     ed_tflops = L * (2 * m + 6) / (ed_ms / 1e3) / 1e12
-    issue = L * m * 4 / (ed_ms / 1e3) / (256 * 4 * 32 * 2.4e9)
+    issue = L * m / 64 * 8.0 / (ed_ms / 1e3) / (256 * 4 * 2.4e9)
     mem["ed_profile"]["roofline"] = {"bound": "valu", "achieved": round(ed_tflops, 2),
                                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                                      "frac": round(ed_tflops / PEAK_FP32_TFLOPS, 4),
