@@ -75,7 +75,11 @@ __device__ __forceinline__ void pair_sync() {
 // the query values qc[0..W) (wave-uniform scalars, loaded in chunks a chunk
 // ahead), z of the column, and the first cell's upper neighbour `prev`.
 // E[W] is the lower wave's cell t = W of the previous column (the right
-// neighbour of its cell W-1), or +inf in the upper wave.
+// neighbour of its cell W-1), or +inf in the upper wave.  No scalar load is
+// left in flight at the end: the barrier that follows waits for every
+// outstanding scalar and LDS access (one counter covers both), so a chunk
+// requested for the next step here measured slower (W = 138..240: 14.2-15.9
+// -> 12.8-15.3 T cells/s at M = 1024).
 template <int W, bool L2>
 __device__ __forceinline__ void pair_cells(float (&E)[W + 1], const float* __restrict__ qc,
                                            float zj, float prev) {
@@ -146,6 +150,10 @@ __device__ __forceinline__ float pair_dp(PairLds& sh, bool lower, const float* _
         float thr = kInf;
         if (check && lower) thr = mode == kModeBest ? load_best_dist(ctrl) : load_thr(ctrl);
         if (j >= 0) {
+            // the lower's cell t = W: requested before the cells, so that
+            // its wait is not between the two barriers (where the upper
+            // wave idles)
+            const float qw = qx[j + W];
             const float zj = band_z(xv, mu, sg, y, fast);
             xv = x[j + 1 < m ? j + 1 : m - 1];
             const float prev = lower ? kInf : sh.lo_last[lane];
@@ -158,7 +166,7 @@ __device__ __forceinline__ float pair_dp(PairLds& sh, bool lower, const float* _
                 pair_sync();
                 const float r = j > 0 ? sh.up_first[lane] : kInf;
                 const float mn = fminf(fminf(E[W - 1], E[W]), r);
-                const float d = qx[j + W] - zj;
+                const float d = qw - zj;
                 E[W] = L2 ? fmaf(d, d, mn) : fabsf(d) + mn;
                 sh.lo_last[lane] = E[W];
                 if (check) {

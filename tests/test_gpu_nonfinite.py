@@ -1,4 +1,4 @@
-"""Non-finite samples in the stream (SURVEY: robustness; the reference leaves
+"""Non-finite samples in the stream (a robustness case the reference leaves
 them undefined).  The rule, in the FP64 and raw statistics modes: a window
 holding a NaN or an infinity has NaN mean and sigma, hence NaN distances and
 bounds -- never a match, never kept by a top-k bound -- and every other
