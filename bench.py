@@ -44,6 +44,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 vector (packed)
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 PEAK_HBM_GBS = 8000.0
 
 
@@ -255,18 +256,18 @@ def section_c2(eng, torch, args):
         gbps = L * nbytes / (ms / 1e3) / 1e9
         mem[name] = {"ms": round(ms, 3), "gb_per_s": round(gbps, 1),
                      "frac_of_8tbs": round(gbps / 8000.0, 3), "bytes_per_window": nbytes}
-    # the ED profile against its own roof: 2M + 6 flop per window (one
-    # multiply-add per element of the correlation form plus the window's
-    # normalisation), and the VALU issue its element sequence needs (sub, fma,
-    # fma: 8.0 SIMD cycles per wave64 element, timed on this device) against
+    # the ED profile against its own roof: it sums in FP64, one fused
+    # multiply-add per window-element (2M flop per window) against the FP64
+    # vector peak, and the issue its loop needs (19 full-rate FP64 operations
+    # per 16 window-elements, 4 SIMD cycles per wave64 operation) against
     # the chip's 256 CU x 4 SIMD at 2.4 GHz
     ed_ms = mem["ed_profile"]["ms"]
-    ed_tflops = L * (2 * m + 6) / (ed_ms / 1e3) / 1e12
-    issue = L * m / 64 * 8.0 / (ed_ms / 1e3) / (256 * 4 * 2.4e9)
+    ed_tflops = L * 2 * m / (ed_ms / 1e3) / 1e12
+    issue = L * m / 64 * (19.0 / 16.0) * 4.0 / (ed_ms / 1e3) / (256 * 4 * 2.4e9)
     mem["ed_profile"]["roofline"] = {"bound": "valu", "achieved": round(ed_tflops, 2),
-                                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                                     "frac": round(ed_tflops / PEAK_FP32_TFLOPS, 4),
-                                     "flops_per_window": 2 * m + 6,
+                                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s (FP64)",
+                                     "frac": round(ed_tflops / PEAK_FP64_TFLOPS, 4),
+                                     "flops_per_window": 2 * m,
                                      "valu_issue_frac": round(issue, 3)}
     # a plain device copy of the stream (torch's copy kernel): the HBM rate a
     # 1 read : 1 write pattern reaches here, beside the kernels above

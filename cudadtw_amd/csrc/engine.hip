@@ -433,6 +433,7 @@ int mdtw_ed(mdtw_engine* e, const float* d_series, const float* d_avg, const flo
     a.mode = kModeProfile;
     a.ctrl = e->d_ctrl;
     a.out = d_out;
+    a.ed = ed_query_sums(e->h_q.data(), e->m);
     return launch_ed(e->stream, e->cus, a);
 }
 

@@ -143,6 +143,15 @@ inline size_t carve_size(std::initializer_list<size_t> bytes) {
     return t + 256;
 }
 
+// FP64 sums of the query the ED kernels use: its mean, and the sum and sum
+// of squares of its centred values (q_k - mean), k ascending.
+struct EdQuerySums {
+    double mean = 0.0;
+    double sum = 0.0;
+    double sumsq = 0.0;
+};
+EdQuerySums ed_query_sums(const float* h_q, int32_t m);
+
 // ---- kernel interface shared by the DP-style kernels ----------------------
 enum BandMode : int {
     kModeProfile = 0,  // out[g] = distance of window g
@@ -176,6 +185,12 @@ struct BandArgs {
     const float* lbt = nullptr;
     const float* lo = nullptr;
     const float* up = nullptr;
+    // ED kernels: the query's sums (ed_query_sums of the query in qx), and
+    // whether the statistics are the search's own FP64 or raw ones (sum of
+    // z^2 over a window = M up to rounding), which the FP32 screen's rounding
+    // margin assumes.
+    EdQuerySums ed;
+    int32_t ed_screen = 0;
 };
 
 struct BandScalars {
@@ -213,6 +228,7 @@ struct SearchPlan {
     const float* up = nullptr;
     const float* qx = nullptr;     // padded query of the register-band kernel
     int32_t qx_len = 0;
+    EdQuerySums ed;                // ED: ed_query_sums of the query
     int32_t m = 0;
     int32_t w = 0;                 // band, <= m - 1
     int32_t algo = MDTW_ALGO_CDTW;

@@ -150,6 +150,7 @@ extern "C" int mdtw_search_noalloc(mdtw_engine* e, const float* d_series, int64_
     p.up = e->d_up;
     p.qx = cdtw ? qx : nullptr;
     p.qx_len = static_cast<int32_t>(qx_len);
+    if (algo == MDTW_ALGO_ED) p.ed = ed_query_sums(e->h_q.data(), m);
     p.m = m;
     p.w = wc;
     p.algo = algo;

@@ -163,6 +163,7 @@ int run_algo(const SearchPlan& p, const BandArgs& a) {
         return launch_dp(p.st, p.cus, a, p.q, p.w, p.metric, false, p.shape);
     BandArgs b = a;
     b.qx = p.q;
+    b.ed = p.ed;
     if (p.algo == MDTW_ALGO_LDTW) return launch_ldtw(p.st, p.cus, b, p.w, p.metric);
     return launch_ed(p.st, p.cus, b);
 }
@@ -233,6 +234,7 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
     a.m = m;
     a.ctrl = p.ctrl;
     a.res = res;
+    a.ed_screen = own_stats && p.stats_mode != MDTW_STATS_REFCOMPAT;
 
     // 2. seed
     if (p.seed) {
@@ -374,6 +376,7 @@ extern "C" int mdtw_search(mdtw_engine* e, const float* d_series, int64_t n, con
     p.up = e->d_up;
     p.qx = e->d_qx;
     p.qx_len = e->qx_len;
+    if (algo == MDTW_ALGO_ED) p.ed = ed_query_sums(e->h_q.data(), m);
     p.m = m;
     p.w = wc;
     p.algo = algo;

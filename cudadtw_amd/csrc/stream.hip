@@ -96,6 +96,7 @@ struct mdtw_stream {
     float* d_up = nullptr;
     float* d_qx = nullptr;
     int32_t qx_len = 0;
+    EdQuerySums ed;  // ED: sums of the query the stream was created with
     char* d_ws = nullptr;
     Ctrl* d_ctrl = nullptr;
     Ctrl* h_ctrl = nullptr;  // pinned
@@ -159,6 +160,7 @@ int upload_query(mdtw_stream* s) {
     const int32_t m = s->m, w = s->w;
     std::vector<float> lo(m), up(m);
     host_envelope(e->h_q.data(), w, lo.data(), up.data(), m);
+    s->ed = ed_query_sums(e->h_q.data(), m);
     const size_t len = static_cast<size_t>(m) + 2 * static_cast<size_t>(w) + kBandChunk + 8;
     std::vector<float> qx(len, kInf);
     for (int32_t i = 0; i < m; ++i) qx[static_cast<size_t>(i) + w] = e->h_q[i];
@@ -356,6 +358,7 @@ int run_epoch(mdtw_stream* s, int64_t nsamples) {
     p.up = s->d_up;
     p.qx = s->d_qx;
     p.qx_len = s->qx_len;
+    p.ed = s->ed;
     p.m = s->m;
     p.w = s->w;
     p.algo = s->algo;

@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 import torch
 
-from cudadtw_amd import ALGO_ED, METRIC_L1, METRIC_L2, STATS_FP64, STATS_REFCOMPAT, _lib, synth
+from cudadtw_amd import (ALGO_ED, METRIC_L1, METRIC_L2, STATS_FP64, STATS_NONE, STATS_REFCOMPAT,
+                         _lib, synth)
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -278,16 +279,23 @@ def test_ed_vs_fp64(engine, m):
     assert int(np.argmin(got)) == int(np.argmin(ref))
 
 
-@pytest.mark.parametrize("n,m", [(300, 256), (1000, 256), (1 << 15, 16), (40_000, 1000),
-                                 (7777, 333), (70_000, 2048)])
-def test_ed_four_window_kernel_equals_per_lane_kernel(engine, n, m):
-    # A 16-byte aligned stream takes the four-windows-per-lane kernel for its
-    # interior tiles (and the per-lane kernel for the last windows); the same
-    # samples one float off alignment take the per-lane kernel throughout.
-    # Both do the same arithmetic per element: bit-identical profiles.
-    s = synth.ecg_like(n, m)
-    q = synth.ecg_query(m, 2) if m >= 200 else synth.random_walk(m, 4)
-    zq = O.znormalize_f32(q)
+@pytest.mark.parametrize("n,m,kind", [(300, 256, "ecg"), (1000, 256, "ecg"), (1 << 15, 16, "ecg"),
+                                        (40_000, 1000, "ecg"), (7777, 333, "ecg"),
+                                        (70_000, 2048, "ecg"), (50_000, 300, "metal7000"),
+                                        (20_000, 128, "walk")])
+def test_ed_exact_over_its_statistics(engine, n, m, kind):
+    # The ED kernels sum in FP64 (centred query, group-shifted samples), so a
+    # window's distance is the exact direct form over the FP32 statistics it
+    # is given, rounded once to FP32: within one FP32 rounding of the oracle's
+    # FP64 direct sum over those same statistics (and within 1e-5 of the sum
+    # over FP64 statistics, test_ed_vs_fp64).  Windows are grouped by their
+    # index relative to the stream pointer, so a stream one float off 16-byte
+    # alignment gives the same bits.
+    s = (synth.ecg_like(n, m) if kind == "ecg" else synth.random_walk(n, m) if kind == "walk"
+         else synth.metal_like(n, m, offset=7000.0))
+    q = synth.ecg_query(m, 2) if kind == "ecg" and m >= 200 else synth.cut_query(s, m, 4)[0]
+    # (centred first: the FP32 query z-normalisation cancels at a level of 7000)
+    zq = O.znormalize_f32((q - q.astype(np.float64).mean()).astype(np.float32))
     engine.set_query(zq, 0)
     ds = dev(s)
     padded = dev(np.concatenate([[0.0], s]).astype(np.float32))
@@ -297,14 +305,46 @@ def test_ed_four_window_kernel_equals_per_lane_kernel(engine, n, m):
     a = host(engine.ed(ds, avg, sd))
     b = host(engine.ed(ds_off, avg, sd))
     assert np.array_equal(a, b)
-    _, _, a64, s64 = O.stats_fp64(s, m, want64=True)
-    ref = O.ed_f64(zq.astype(np.float64), s, a64, s64)
-    assert np.max(np.abs(a - ref) / np.maximum(ref, 1.0)) < 1e-5
-    for k in (1, 4):  # searches through both kernels agree too
-        r1 = engine.search(ds, 0, algo=ALGO_ED, k=k, excl=m // 2)
-        r2 = engine.search(ds_off, 0, algo=ALGO_ED, k=k, excl=m // 2)
-        assert np.array_equal(r1.positions, r2.positions)
-        assert np.array_equal(r1.distances, r2.distances)
+    ref = O.ed_f64(zq.astype(np.float64), s, host(avg).astype(np.float64),
+                   host(sd).astype(np.float64))
+    assert np.all(np.abs(a - ref) <= 1.2e-7 * ref + 1e-12 * m)
+
+
+@pytest.mark.parametrize("m,stats", [(256, "own"), (600, "own"), (1000, "own"), (2048, "own"),
+                                     (600, "given"), (777, "raw")])
+def test_ed_search_kernels_agree_with_profile(engine, m, stats):
+    # Searches evaluate windows with other kernels than the profile: strided
+    # seeds one window per lane, and for queries of 512 or more over the
+    # search's own statistics an FP32 screen followed by the exact evaluation
+    # of the windows it keeps.  The 1024 best windows (no exclusion zone) and
+    # the best match must carry the profile's bits, also with non-finite
+    # samples planted inside the groups of the best windows (their
+    # neighbours then sum their own samples).
+    n = 60_000
+    s = synth.ecg_like(n, m)
+    q, pos = synth.cut_query(s, m, 3, noise=0.02)
+    s = s.copy()
+    for off in (-9, 5 + m, 3 * m + 7):  # near the cut, outside its window
+        if 0 <= pos + off < n:
+            s[pos + off] = np.nan if off < 0 else np.inf
+    ds = dev(s)
+    if stats == "raw":
+        zq = q.astype(np.float32)
+        engine.set_query(zq, 0)
+        avg, sd = engine.avg_std(ds, m, STATS_NONE)
+        kw = dict(stats_mode=STATS_NONE)
+    else:
+        zq = O.znormalize_f32(q)
+        engine.set_query(zq, 0)
+        avg, sd = engine.avg_std(ds, m)
+        kw = dict(avg=avg, sd=sd) if stats == "given" else {}
+    prof = host(engine.ed(ds, avg, sd))
+    assert np.isnan(prof[pos - 9 - m + 1:pos - 9 + 1]).all()
+    for k, excl in ((1024, 0), (1, 0), (8, m // 2)):
+        r = engine.search(ds, 0, algo=ALGO_ED, k=k, excl=excl, **kw)
+        idx, d2 = O.topk_greedy(prof, k, excl)
+        np.testing.assert_array_equal(r.positions[:idx.size], idx)
+        np.testing.assert_array_equal(r.distances[:idx.size], np.sqrt(d2).astype(np.float32))
 
 
 # ---- primitives (tests/test_threshold.cu) -------------------------------------------------------
