@@ -10,17 +10,20 @@
 // Overlap-save in blocks of N real samples (N = the smallest power of two
 // >= 2M, at least 4096, at most 16384: M <= 8192): block b holds samples
 // [bV, bV + N) with V = N - M + 1 windows, minus the block's first sample c
-// (offset data loses no precision).  One workgroup of 1024 threads per
-// block, all in LDS (H = N/2 complex doubles, at most 128 KiB):
+// (offset data loses no precision).  One workgroup of H/8 threads per
+// block, all in LDS (H = N/2 complex doubles plus one pad per 8, at most
+// 144 KiB):
 //   - the N real samples are packed as H complex values z[n] = x[2n] +
 //     i x[2n+1], stored at bit-reversed positions,
-//   - forward radix-2 decimation-in-time FFT of size H, in place,
+//   - forward decimation-in-time FFT of size H, in place, three radix-2
+//     stages per pass in registers (one LDS round trip and one barrier per
+//     three stages),
 //   - real-spectrum split X[k] = E[k] + W^k O[k], E/O from Z[k] and
 //     conj(Z[H-k]) (pairs k, H-k in one thread), product Y = X conj(Q)
 //     with the query's spectrum Q (same transform, computed once), and the
 //     inverse's packing A[k] + i B[k], A = Y[k] + conj(Y[H-k]),
 //     B = (Y[k] - conj(Y[H-k])) conj(W^k),
-//   - inverse radix-2 decimation-in-frequency FFT (natural order in,
+//   - inverse decimation-in-frequency FFT, three stages per pass (natural order in,
 //     bit-reversed out), so y[2n] + i y[2n+1] = z'[n] / N is the
 //     cross-correlation dot_r = sum_i q_i (x_{bV+r+i} - c), r < V,
 //   - ED^2 = sum q^2 + sum z^2 - 2 (dot_r - (mu_r - c) sum q) / sigma_r,
@@ -42,7 +45,6 @@ using namespace mdtw;
 
 namespace {
 
-constexpr int kFftThreads = 1024;
 constexpr int kFftMinLog = 12;  // N >= 4096 real samples per block
 constexpr int kFftMaxLog = 14;  // N <= 16384 (H = 8192 complex doubles in LDS)
 
@@ -68,42 +70,121 @@ __global__ __launch_bounds__(256) void twiddle_kernel(double2* __restrict__ tw, 
     tw[j] = {cs, -sn};
 }
 
+// LDS layout of the H complex values: one pad entry after every 8, so the
+// eight-element groups of the first radix-8 pass (lane strides of 8
+// entries) and the strided groups of the later ones fall in distinct banks.
+__device__ __forceinline__ int fx(int i) { return i + (i >> 3); }
+template <int LOGH>
+constexpr int fft_lds_entries() {
+    return (1 << LOGH) + (1 << LOGH) / 8;
+}
+// Threads per transform block: one radix-8 group each.
+template <int LOGH>
+constexpr int fft_threads() {
+    return (1 << LOGH) / 8;
+}
+
+__device__ __forceinline__ void bfly_dit(double2& u, double2& v, double2 w) {
+    const double2 t = cmul(v, w);
+    v = csub(u, t);
+    u = cadd(u, t);
+}
+__device__ __forceinline__ void bfly_dif(double2& u, double2& v, double2 w) {
+    const double2 d = csub(u, v);
+    u = cadd(u, v);
+    v = cmul(d, cconj(w));
+}
+
 // In-place forward DIT FFT of a[0..H) (bit-reversed in, natural out).
+// Stage s combines pairs (i, i + 2^(s-1)) within blocks of 2^s with the
+// twiddle of i's position in its block; three consecutive stages s..s+2 only
+// combine the eight entries pos + k 2^(s-1) (k < 8) of a block of 2^(s+2),
+// so a thread takes those eight into registers, applies the three stages
+// (the same butterflies, in the same order per entry, as stage by stage) and
+// writes them back: one LDS round trip and one barrier per three stages.
 template <int LOGH>
 __device__ __forceinline__ void fft_dit(double2* a, const double2* __restrict__ tw) {
     constexpr int H = 1 << LOGH;
-    for (int s = 1; s <= LOGH; ++s) {
+    constexpr int NT = fft_threads<LOGH>();
+    int s = 1;
+    for (; s + 2 <= LOGH; s += 3) {
+        const int h = 1 << (s - 1);
+        for (int b = threadIdx.x; b < H / 8; b += NT) {
+            const int pos = b & (h - 1);
+            const int base = ((b >> (s - 1)) << (s + 2)) + pos;
+            double2 e[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) e[k] = a[fx(base + k * h)];
+            const double2 w1 = tw[pos << (LOGH - s + 1)];
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) bfly_dit(e[k], e[k + 1], w1);
+            const double2 w2a = tw[pos << (LOGH - s)], w2b = tw[(pos + h) << (LOGH - s)];
+            bfly_dit(e[0], e[2], w2a);
+            bfly_dit(e[1], e[3], w2b);
+            bfly_dit(e[4], e[6], w2a);
+            bfly_dit(e[5], e[7], w2b);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) bfly_dit(e[k], e[k + 4], tw[(pos + k * h) << (LOGH - s - 1)]);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a[fx(base + k * h)] = e[k];
+        }
+        __syncthreads();
+    }
+    for (; s <= LOGH; ++s) {  // the remaining one or two stages, radix 2
         const int half = 1 << (s - 1);
-        for (int b = threadIdx.x; b < H / 2; b += kFftThreads) {
+        for (int b = threadIdx.x; b < H / 2; b += NT) {
             const int pos = b & (half - 1);
             const int i = ((b >> (s - 1)) << s) + pos;
-            const int j = i + half;
-            const double2 w = tw[pos << (LOGH - s + 1)];
-            const double2 u = a[i];
-            const double2 v = cmul(a[j], w);
-            a[i] = cadd(u, v);
-            a[j] = csub(u, v);
+            double2 u = a[fx(i)], v = a[fx(i + half)];
+            bfly_dit(u, v, tw[pos << (LOGH - s + 1)]);
+            a[fx(i)] = u;
+            a[fx(i + half)] = v;
         }
         __syncthreads();
     }
 }
 
 // In-place inverse DIF FFT of a[0..H) (natural in, bit-reversed out),
-// unnormalised: sum_k a[k] exp(+2 pi i n k / H).
+// unnormalised: sum_k a[k] exp(+2 pi i n k / H).  Stages s = LOGH .. 1
+// (pairs (i, i + 2^(s-1)), difference times the conjugate twiddle of i's
+// position), three per LDS round trip as in fft_dit.
 template <int LOGH>
 __device__ __forceinline__ void ifft_dif(double2* a, const double2* __restrict__ tw) {
     constexpr int H = 1 << LOGH;
-    for (int s = LOGH; s >= 1; --s) {
+    constexpr int NT = fft_threads<LOGH>();
+    int s = LOGH;
+    for (; s - 2 >= 1; s -= 3) {
+        const int h = 1 << (s - 3);
+        for (int b = threadIdx.x; b < H / 8; b += NT) {
+            const int pos = b & (h - 1);
+            const int base = ((b >> (s - 3)) << s) + pos;
+            double2 e[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) e[k] = a[fx(base + k * h)];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) bfly_dif(e[k], e[k + 4], tw[(pos + k * h) << (LOGH - s + 1)]);
+            const double2 w2a = tw[pos << (LOGH - s + 2)], w2b = tw[(pos + h) << (LOGH - s + 2)];
+            bfly_dif(e[0], e[2], w2a);
+            bfly_dif(e[1], e[3], w2b);
+            bfly_dif(e[4], e[6], w2a);
+            bfly_dif(e[5], e[7], w2b);
+            const double2 w1 = tw[pos << (LOGH - s + 3)];
+#pragma unroll
+            for (int k = 0; k < 8; k += 2) bfly_dif(e[k], e[k + 1], w1);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) a[fx(base + k * h)] = e[k];
+        }
+        __syncthreads();
+    }
+    for (; s >= 1; --s) {
         const int half = 1 << (s - 1);
-        for (int b = threadIdx.x; b < H / 2; b += kFftThreads) {
+        for (int b = threadIdx.x; b < H / 2; b += NT) {
             const int pos = b & (half - 1);
             const int i = ((b >> (s - 1)) << s) + pos;
-            const int j = i + half;
-            const double2 w = cconj(tw[pos << (LOGH - s + 1)]);
-            const double2 u = a[i];
-            const double2 v = a[j];
-            a[i] = cadd(u, v);
-            a[j] = cmul(csub(u, v), w);
+            double2 u = a[fx(i)], v = a[fx(i + half)];
+            bfly_dif(u, v, tw[pos << (LOGH - s + 1)]);
+            a[fx(i)] = u;
+            a[fx(i + half)] = v;
         }
         __syncthreads();
     }
@@ -122,26 +203,27 @@ __device__ __forceinline__ void real_split(double2 zk, double2 zm, double2 wk, d
 // The query's real spectrum Q[0..H] (q zero-padded to N = 2H); QT is the
 // query's type (FP32, or FP64 for the FP64-stream form).
 template <int LOGH, class QT>
-__global__ __launch_bounds__(kFftThreads) void query_spectrum_kernel(
+__global__ __launch_bounds__(fft_threads<LOGH>()) void query_spectrum_kernel(
     const QT* __restrict__ q, int32_t m, const double2* __restrict__ tw,
     double2* __restrict__ qs) {
     constexpr int H = 1 << LOGH;
-    __shared__ double2 a[H];
-    for (int n = threadIdx.x; n < H; n += kFftThreads) {
+    constexpr int NT = fft_threads<LOGH>();
+    __shared__ double2 a[fft_lds_entries<LOGH>()];
+    for (int n = threadIdx.x; n < H; n += NT) {
         const double v0 = 2 * n < m ? static_cast<double>(q[2 * n]) : 0.0;
         const double v1 = 2 * n + 1 < m ? static_cast<double>(q[2 * n + 1]) : 0.0;
-        a[bitrev<LOGH>(n)] = {v0, v1};
+        a[fx(bitrev<LOGH>(n))] = {v0, v1};
     }
     __syncthreads();
     fft_dit<LOGH>(a, tw);
-    for (int k = threadIdx.x; k <= H / 2; k += kFftThreads) {
+    for (int k = threadIdx.x; k <= H / 2; k += NT) {
         if (k == 0) {
-            const double2 z0 = a[0];
+            const double2 z0 = a[fx(0)];
             qs[0] = {z0.x + z0.y, 0.0};
             qs[H] = {z0.x - z0.y, 0.0};
         } else {
             double2 xk, xm;
-            real_split(a[k], a[H - k], tw[k], tw[H - k], &xk, &xm);
+            real_split(a[fx(k)], a[fx(H - k)], tw[k], tw[H - k], &xk, &xm);
             qs[k] = xk;
             qs[H - k] = xm;  // k = H/2: the same value twice
         }
@@ -161,50 +243,51 @@ struct FftScalars {
 // S: the stream's type, O: the profile's (FP32 for FP32 streams, FP64 for
 // FP64 ones).
 template <int LOGH, class S, class O>
-__global__ __launch_bounds__(kFftThreads) void ed_fft_kernel(
+__global__ __launch_bounds__(fft_threads<LOGH>()) void ed_fft_kernel(
     const S* __restrict__ s, const double* __restrict__ avg, const double* __restrict__ var,
     const double2* __restrict__ tw, const double2* __restrict__ qs, O* __restrict__ out,
     FftScalars a) {
     constexpr int H = 1 << LOGH;
     constexpr int N = 2 * H;
-    __shared__ double2 z[H];
+    constexpr int NT = fft_threads<LOGH>();
+    __shared__ double2 z[fft_lds_entries<LOGH>()];
     const int64_t base = static_cast<int64_t>(blockIdx.x) * a.V;
     // Non-finite samples enter the transform as 0 (their windows' statistics
     // are NaN, so those windows' distances are NaN); one would otherwise
     // spread through the whole block's transform.
     const double c = isfinite(s[base]) ? static_cast<double>(s[base]) : 0.0;
-    for (int n = threadIdx.x; n < H; n += kFftThreads) {
+    for (int n = threadIdx.x; n < H; n += NT) {
         const int64_t i0 = base + 2 * n;
         const S x0 = i0 < a.n ? s[i0] : S(0);
         const S x1 = i0 + 1 < a.n ? s[i0 + 1] : S(0);
         const double v0 = isfinite(x0) && i0 < a.n ? static_cast<double>(x0) - c : 0.0;
         const double v1 = isfinite(x1) && i0 + 1 < a.n ? static_cast<double>(x1) - c : 0.0;
-        z[bitrev<LOGH>(n)] = {v0, v1};
+        z[fx(bitrev<LOGH>(n))] = {v0, v1};
     }
     __syncthreads();
     fft_dit<LOGH>(z, tw);
-    for (int k = threadIdx.x; k <= H / 2; k += kFftThreads) {
+    for (int k = threadIdx.x; k <= H / 2; k += NT) {
         if (k == 0) {
-            const double2 z0 = z[0];
+            const double2 z0 = z[fx(0)];
             const double2 y0 = cmul({z0.x + z0.y, 0.0}, cconj(qs[0]));
             const double2 yh = cmul({z0.x - z0.y, 0.0}, cconj(qs[H]));
             const double2 A = cadd(y0, cconj(yh));
             const double2 B = csub(y0, cconj(yh));
-            z[0] = {A.x - B.y, A.y + B.x};
+            z[fx(0)] = {A.x - B.y, A.y + B.x};
         } else {
             const int km = H - k;
             const double2 wk = tw[k], wm = tw[km];
             double2 xk, xm;
-            real_split(z[k], z[km], wk, wm, &xk, &xm);
+            real_split(z[fx(k)], z[fx(km)], wk, wm, &xk, &xm);
             const double2 yk = cmul(xk, cconj(qs[k]));
             const double2 ym = cmul(xm, cconj(qs[km]));
             const double2 Ak = cadd(yk, cconj(ym));
             const double2 Bk = cmul(csub(yk, cconj(ym)), cconj(wk));
-            z[k] = {Ak.x - Bk.y, Ak.y + Bk.x};
+            z[fx(k)] = {Ak.x - Bk.y, Ak.y + Bk.x};
             if (km != k) {
                 const double2 Am = cadd(ym, cconj(yk));
                 const double2 Bm = cmul(csub(ym, cconj(yk)), cconj(wm));
-                z[km] = {Am.x - Bm.y, Am.y + Bm.x};
+                z[fx(km)] = {Am.x - Bm.y, Am.y + Bm.x};
             }
         }
     }
@@ -212,10 +295,10 @@ __global__ __launch_bounds__(kFftThreads) void ed_fft_kernel(
     ifft_dif<LOGH>(z, tw);
     const double inv_n = 1.0 / static_cast<double>(N);
     const double m = static_cast<double>(a.m);
-    for (int r = threadIdx.x; r < a.V; r += kFftThreads) {
+    for (int r = threadIdx.x; r < a.V; r += NT) {
         const int64_t p = base + r;
         if (p >= a.L) break;
-        const double2 v = z[bitrev<LOGH>(r >> 1)];
+        const double2 v = z[fx(bitrev<LOGH>(r >> 1))];
         const double dot = ((r & 1) ? v.y : v.x) * inv_n;  // sum q (x - c)
         const double mu = avg[p], va = var[p];
         const double sig = va > 0.0 ? sqrt(va) : 1.0;
@@ -287,13 +370,14 @@ int fft_log(int32_t m) {
 template <int LOGH, class S, class O>
 int launch_fft_pair(hipStream_t st, const S* s, const double* avg, const double* var,
                     const double2* tw, const S* q, double2* qs, O* out, const FftScalars& a) {
-    hipLaunchKernelGGL((query_spectrum_kernel<LOGH, S>), dim3(1), dim3(kFftThreads), 0, st, q,
+    constexpr int NT = fft_threads<LOGH>();
+    hipLaunchKernelGGL((query_spectrum_kernel<LOGH, S>), dim3(1), dim3(NT), 0, st, q,
                        a.m, tw, qs);
     int rc = hip_ok(hipGetLastError());
     if (rc != MDTW_OK) return rc;
     const int64_t nb = (a.L + a.V - 1) / a.V;
     hipLaunchKernelGGL((ed_fft_kernel<LOGH, S, O>), dim3(static_cast<unsigned>(nb)),
-                       dim3(kFftThreads), 0, st, s, avg, var, tw, qs, out, a);
+                       dim3(NT), 0, st, s, avg, var, tw, qs, out, a);
     return hip_ok(hipGetLastError());
 }
 
