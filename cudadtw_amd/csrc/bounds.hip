@@ -16,10 +16,16 @@
 namespace mdtw {
 namespace {
 
-template <bool EXACT>
+// z-normalisation modes: kZnRecip (x - mu) * (1/sigma), the cascade's
+// (pruning margin); kZnDiv the IEEE quotient; kZnMark the same quotient
+// through band_z_fast (rs = 1/sigma), for windows band_z_window_fast admits.
+enum ZnMode { kZnRecip = 0, kZnDiv = 1, kZnMark = 2 };
+template <int Z>
 __device__ __forceinline__ float zn(float x, float mu, float sg, float rs) {
-    if constexpr (EXACT)
+    if constexpr (Z == kZnDiv)
         return (x - mu) / sg;
+    else if constexpr (Z == kZnMark)
+        return band_z_fast(x, mu, sg, rs);
     else
         return (x - mu) * rs;
 }
@@ -47,7 +53,7 @@ __device__ __forceinline__ float lpa(float d, float acc) {
 // cost + min(left, min(diagonal, up)), cost then min in that order, so the
 // rounding matches the CPU restatement; the tile is fully unrolled into
 // registers (five stream values, one row of cells).
-template <bool L2, bool EXACT, bool FRONT>
+template <bool L2, int EXACT, bool FRONT>
 __device__ __forceinline__ float kim_corner(const float* __restrict__ x,
                                             const float* __restrict__ q, float mu, float sg,
                                             float rs, float* exit_min) {
@@ -87,7 +93,7 @@ __device__ __forceinline__ float kim_corner(const float* __restrict__ x,
     return row[4];
 }
 
-template <bool L2, bool EXACT>
+template <bool L2, int EXACT>
 __device__ __forceinline__ float lb_kim_window(const float* __restrict__ x,
                                                const float* __restrict__ q, int32_t m,
                                                float mu, float sg, float rs) {
@@ -108,7 +114,53 @@ __global__ __launch_bounds__(256) void lb_kim_kernel(const float* __restrict__ s
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= count) return;
     const int64_t w = idx ? idx[g] : g;
-    out[g] = lb_kim_window<L2, true>(s + w, q, m, avg[w], sd[w], 0.0f);
+    const float mu = avg[w], sg = sd[w];
+    // the ten quotients without a division when the whole wave's windows allow it
+    if (__all(band_z_window_fast(mu, sg)))
+        out[g] = lb_kim_window<L2, kZnMark>(s + w, q, m, mu, sg, 1.0f / sg);
+    else
+        out[g] = lb_kim_window<L2, kZnDiv>(s + w, q, m, mu, sg, 0.0f);
+}
+
+// LB_Keogh of one window, z = (x - mu) / sigma as the exact IEEE quotient
+// without a division per element: band_z, or band_z_fast (no per-element
+// test) when FASTZ.  The envelope index is wave-uniform, so blocks of 16
+// read it with scalar loads.
+template <bool L2, bool FASTZ>
+__device__ __forceinline__ float keogh_window(const float* __restrict__ x, float mu, float sg,
+                                              const float* __restrict__ lo,
+                                              const float* __restrict__ up, int32_t m) {
+    const float y = 1.0f / sg;
+    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
+    auto zof = [&](float v) { return FASTZ ? band_z_fast(v, mu, sg, y) : band_z(v, mu, sg, y, fast); };
+    float acc = 0.0f;
+    int32_t i = 0;
+    for (; i + 16 <= m; i += 16) {
+        const float* __restrict__ lq = lo + i;
+        const float* __restrict__ uq = up + i;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const float z = zof(x[i + t]);
+            const float e = z - __builtin_amdgcn_fmed3f(z, lq[t], uq[t]);
+            acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+        }
+    }
+    for (; i < m; ++i) {
+        const float z = zof(x[i]);
+        const float e = z - __builtin_amdgcn_fmed3f(z, lo[i], up[i]);
+        acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
+    }
+    return acc;
+}
+
+template <bool L2>
+__device__ __forceinline__ float keogh_window_any(const float* __restrict__ x, float mu, float sg,
+                                                  const float* __restrict__ lo,
+                                                  const float* __restrict__ up, int32_t m) {
+    // wave-uniform choice: the loop without per-element tests when every
+    // active lane's window admits it (almost always)
+    if (__all(band_z_window_fast(mu, sg))) return keogh_window<L2, true>(x, mu, sg, lo, up, m);
+    return keogh_window<L2, false>(x, mu, sg, lo, up, m);
 }
 
 // bounds.cuh:111-144.  e = z - clamp(z, lo, up) is the reference's `upper`
@@ -121,31 +173,7 @@ __global__ __launch_bounds__(256) void lb_keogh_kernel(
     const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (g >= count) return;
     const int64_t w = idx ? idx[g] : g;
-    const float* x = s + w;
-    const float mu = avg[w], sg = sd[w];
-    // z = (x - mu) / sigma as the exact IEEE quotient without a division
-    // per element (band_z); the envelope index is wave-uniform, so blocks of
-    // 16 read it with scalar loads.
-    const float y = 1.0f / sg;
-    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
-    float acc = 0.0f;
-    int32_t i = 0;
-    for (; i + 16 <= m; i += 16) {
-        const float* __restrict__ lq = lo + i;
-        const float* __restrict__ uq = up + i;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const float z = band_z(x[i + t], mu, sg, y, fast);
-            const float e = z - __builtin_amdgcn_fmed3f(z, lq[t], uq[t]);
-            acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
-        }
-    }
-    for (; i < m; ++i) {
-        const float z = band_z(x[i], mu, sg, y, fast);
-        const float e = z - __builtin_amdgcn_fmed3f(z, lo[i], up[i]);
-        acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
-    }
-    out[g] = acc;
+    out[g] = keogh_window_any<L2>(s + w, avg[w], sd[w], lo, up, m);
 }
 
 // All windows: a workgroup's 256 consecutive windows read samples
@@ -168,28 +196,7 @@ __global__ __launch_bounds__(256) void lb_keogh_all_kernel(
     __syncthreads();
     const int64_t g = g0 + threadIdx.x;
     if (g >= count) return;
-    const float* x = seg + threadIdx.x;
-    const float mu = avg[g], sg = sd[g];
-    const float y = 1.0f / sg;
-    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
-    float acc = 0.0f;
-    int32_t i = 0;
-    for (; i + 16 <= m; i += 16) {
-        const float* __restrict__ lq = lo + i;
-        const float* __restrict__ uq = up + i;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const float z = band_z(x[i + t], mu, sg, y, fast);
-            const float e = z - __builtin_amdgcn_fmed3f(z, lq[t], uq[t]);
-            acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
-        }
-    }
-    for (; i < m; ++i) {
-        const float z = band_z(x[i], mu, sg, y, fast);
-        const float e = z - __builtin_amdgcn_fmed3f(z, lo[i], up[i]);
-        acc = L2 ? fmaf(e, e, acc) : acc + fabsf(e);
-    }
-    out[g] = acc;
+    out[g] = keogh_window_any<L2>(seg + threadIdx.x, avg[g], sd[g], lo, up, m);
 }
 
 __device__ __forceinline__ uint32_t lane_rank(unsigned long long mask) {
@@ -292,7 +299,7 @@ __global__ __launch_bounds__(kCascadeWaves * kWave) void cascade_kernel(
         const float* x = s + wc;
         const float mu = avg[wc], sg = sd[wc];
         const float rs = 1.0f / sg;
-        const float kim = lb_kim_window<L2, false>(x, q, m, mu, sg, rs);
+        const float kim = lb_kim_window<L2, kZnRecip>(x, q, m, mu, sg, rs);
         bool alive = valid && !(kim > lim);
         n_kim += __popcll(__ballot(valid && !alive));
         if (!__any(alive)) continue;

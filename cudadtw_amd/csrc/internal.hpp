@@ -464,6 +464,21 @@ __device__ __forceinline__ float band_z(float x, float mu, float sg, float y, bo
     return z;
 }
 
+// band_z's fast path for every sample of a window, decided once per window:
+// sigma in range, and either |mu| >= 2^-75 (then a nonzero x - mu of FP32
+// values is at least 2^-99: both at or above 2^-76 in magnitude, the
+// difference is a multiple of 2^-99; x below 2^-76, it is above 2^-76) or
+// sigma == 1 (q = a * 1 is exact and the residual 0).  Then the quotient is
+// band_z_fast, with no per-sample test.
+__device__ __forceinline__ bool band_z_window_fast(float mu, float sg) {
+    return sg >= 0x1p-100f && sg <= 0x1p100f && (fabsf(mu) >= 0x1p-75f || sg == 1.0f);
+}
+__device__ __forceinline__ float band_z_fast(float x, float mu, float sg, float y) {
+    const float a = x - mu;
+    const float q = a * y;
+    return fmaf(fmaf(-sg, q, a), y, q);
+}
+
 // Exact distance d of window w is known: lower its bucket's minimum.
 __device__ __forceinline__ void note_bucket(uint32_t* bmin, long long bucket_len, int64_t w,
                                             float d) {
