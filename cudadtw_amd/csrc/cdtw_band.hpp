@@ -168,7 +168,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     }
 
     const float y = 1.0f / sg;
-    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
+    const bool wfast = band_z_window_fast(mu, sg);
 
     // Samples of the next U columns; indices are clamped rather than
     // branched so the loads stay unconditional and overlap a whole block.
@@ -209,7 +209,7 @@ __device__ __forceinline__ float band_dp(const float* __restrict__ x, float mu, 
     for (int32_t j0 = 0; j0 < m; j0 += U) {
         float z[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) z[u] = band_z(xv[u], mu, sg, y, fast);
+        for (int u = 0; u < U; ++u) z[u] = band_z_w(xv[u], mu, sg, y, wfast);
         if constexpr (LB) {
             // U divides 64, so a block of U columns never straddles a chunk.
             if (j0 > 0 && (j0 & (kWave - 1)) == 0) {

@@ -136,7 +136,7 @@ __device__ __forceinline__ float pair_dp(PairLds& sh, bool lower, const float* _
     if (lower) E[W] = 0.0f;  // D[-1][-1]
 
     const float y = 1.0f / sg;
-    const bool fast = sg >= 0x1p-100f && sg <= 0x1p100f;
+    const bool wfast = band_z_window_fast(mu, sg);
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     // lower: column s at step s; upper: column s - 1
     const int32_t lag = lower ? 0 : 1;
@@ -154,7 +154,7 @@ __device__ __forceinline__ float pair_dp(PairLds& sh, bool lower, const float* _
             // its wait is not between the two barriers (where the upper
             // wave idles)
             const float qw = qx[j + W];
-            const float zj = band_z(xv, mu, sg, y, fast);
+            const float zj = band_z_w(xv, mu, sg, y, wfast);
             xv = x[j + 1 < m ? j + 1 : m - 1];
             const float prev = lower ? kInf : sh.lo_last[lane];
             pair_cells<W, L2>(E, qx + j + qoff, zj, prev);

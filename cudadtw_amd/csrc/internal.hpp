@@ -478,6 +478,12 @@ __device__ __forceinline__ float band_z_fast(float x, float mu, float sg, float 
     const float q = a * y;
     return fmaf(fmaf(-sg, q, a), y, q);
 }
+// band_z with the window's test hoisted: wfast = band_z_window_fast(mu, sg)
+// (loop-invariant); the rare other windows keep the per-sample test.
+__device__ __forceinline__ float band_z_w(float x, float mu, float sg, float y, bool wfast) {
+    if (__builtin_expect(wfast, 1)) return band_z_fast(x, mu, sg, y);
+    return band_z(x, mu, sg, y, sg >= 0x1p-100f && sg <= 0x1p100f);
+}
 
 // Exact distance d of window w is known: lower its bucket's minimum.
 __device__ __forceinline__ void note_bucket(uint32_t* bmin, long long bucket_len, int64_t w,
