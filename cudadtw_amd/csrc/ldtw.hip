@@ -7,9 +7,9 @@
 // not allowed), with the reference's tie rules: diagonal only if strictly
 // below both others, then down if not above diagonal and strictly below
 // right, else right.  The cost is the sum of the visited cells' local costs.
-// z-normalisation is (x - mu) / sigma (IEEE division), squares and sums are
-// separately rounded (no contraction), so the value is bit-exact with the CPU
-// restatement.  Each z-normalised stream value is computed once and carried
+// z-normalisation is (x - mu) / sigma (the IEEE quotient), squares and sums
+// are separately rounded (no contraction), so the value is bit-exact with the
+// CPU restatement.  Each z-normalised stream value is computed once and carried
 // while the path stays on its column.
 //
 // Search modes abandon a wave once every lane's partial cost (which only
@@ -31,14 +31,25 @@ __device__ __forceinline__ float lcost(float t) {
         return t < 0 ? -t : t;
 }
 
-template <bool L2, bool QL>
+// FASTZ: z = (x - mu) / sigma through band_z_fast (every window of the wave
+// admits it, band_z_window_fast), else the IEEE division; the same quotient.
+template <bool FASTZ>
+__device__ __forceinline__ float ldtw_z(float x, float mu, float sg, float y) {
+    if constexpr (FASTZ)
+        return band_z_fast(x, mu, sg, y);
+    else
+        return (x - mu) / sg;
+}
+
+template <bool L2, bool QL, bool FASTZ>
 __device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float mu, float sg,
                              const float* __restrict__ q, int32_t M, int32_t W, int mode,
                              const Ctrl* ctrl, bool& alive) {
     int32_t i = 0, j = 0;
+    const float y = 1.0f / sg;
     float qi = q[0], qi1 = M > 1 ? q[1] : 0.0f;
-    float zj = (x[0] - mu) / sg;
-    float zj1 = M > 1 ? (x[1] - mu) / sg : 0.0f;
+    float zj = ldtw_z<FASTZ>(x[0], mu, sg, y);
+    float zj1 = M > 1 ? ldtw_z<FASTZ>(x[1], mu, sg, y) : 0.0f;
     float d = lcost<L2>(qi - zj);
     const bool abandon = mode == kModeBest || mode == kModeCollect;
     const float thr = abandon ? load_abandon_bound(ctrl, mode) : kInf;
@@ -95,7 +106,7 @@ __device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float 
         if (adv_j && j + 1 < M) {
             ++j;
             zj = zj1;
-            zj1 = j + 1 < M ? (x[j + 1] - mu) / sg : 0.0f;
+            zj1 = j + 1 < M ? ldtw_z<FASTZ>(x[j + 1], mu, sg, y) : 0.0f;
         }
         ++step;
         if (abandon && (step % kLdtwCheck) == 0) {
@@ -142,7 +153,11 @@ __global__ __launch_bounds__(64) void ldtw_kernel(
         const int64_t gc = valid ? g : count - 1;
         const int64_t w = idx ? static_cast<int64_t>(idx[gc]) : a.base + gc * a.stride;
         bool alive = valid;
-        const float d = ldtw_window<L2, QL>(s + w, avg[w], sd[w], q, a.m, W, a.mode, ctrl, alive);
+        const float mu = avg[w], sg = sd[w];
+        const float d =
+            __all(band_z_window_fast(mu, sg))
+                ? ldtw_window<L2, QL, true>(s + w, mu, sg, q, a.m, W, a.mode, ctrl, alive)
+                : ldtw_window<L2, QL, false>(s + w, mu, sg, q, a.m, W, a.mode, ctrl, alive);
         if (a.mode == kModeProfile || a.mode == kModeSeed) {
             if (valid) out[g] = d;
         }
