@@ -339,9 +339,30 @@ def test_ed_search_kernels_agree_with_profile(engine, m, stats):
         avg, sd = engine.avg_std(ds, m)
         kw = dict(avg=avg, sd=sd) if stats == "given" else {}
     prof = host(engine.ed(ds, avg, sd))
-    assert np.isnan(prof[pos - 9 - m + 1:pos - 9 + 1]).all()
+    assert np.isnan(prof[max(0, pos - 9 - m + 1):pos - 9 + 1]).all()
     for k, excl in ((1024, 0), (1, 0), (8, m // 2)):
         r = engine.search(ds, 0, algo=ALGO_ED, k=k, excl=excl, **kw)
+        idx, d2 = O.topk_greedy(prof, k, excl)
+        np.testing.assert_array_equal(r.positions[:idx.size], idx)
+        np.testing.assert_array_equal(r.distances[:idx.size], np.sqrt(d2).astype(np.float32))
+
+
+def test_ed_topk_rounds_split_inside_a_group(engine):
+    # Top-k searches run a first round over L/16 windows and a second over the
+    # rest; with L = 1,048,600 the second starts at window 65,537, inside a
+    # 16-window group, so both rounds evaluate that group's windows (each
+    # reporting its own).  The kept windows must still carry the profile's bits.
+    m = 256
+    L = 1_048_600
+    s = synth.ecg_like(L + m - 1, 12)
+    q, _ = synth.cut_query(s, m, 13, noise=0.05)
+    zq = O.znormalize_f32(q)
+    engine.set_query(zq, 0)
+    ds = dev(s)
+    avg, sd = engine.avg_std(ds, m)
+    prof = host(engine.ed(ds, avg, sd))
+    for k, excl in ((8, m // 2), (300, 0)):
+        r = engine.search(ds, 0, algo=ALGO_ED, k=k, excl=excl)
         idx, d2 = O.topk_greedy(prof, k, excl)
         np.testing.assert_array_equal(r.positions[:idx.size], idx)
         np.testing.assert_array_equal(r.distances[:idx.size], np.sqrt(d2).astype(np.float32))
