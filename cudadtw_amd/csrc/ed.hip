@@ -337,7 +337,10 @@ __global__ __launch_bounds__(kEdThreads) void ed_tile_kernel(
             }
         }
         best = wave_min_u64(best);
-        if ((tid & (kWave - 1)) == 0 && best != ~0ull) atomicMin(&ctrl->best, best);
+        // keys only fall: skip the same-address atomic when it cannot win
+        if ((tid & (kWave - 1)) == 0 && best != ~0ull &&
+            best < __hip_atomic_load(&ctrl->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMin(&ctrl->best, best);
     } else {
         const float thr = load_thr(ctrl);
 #pragma unroll

@@ -164,7 +164,9 @@ int mdtw_ldtw(mdtw_engine* e, const float* d_series, const float* d_avg,
               const float* d_std, float* d_out, int64_t count, int32_t w,
               int32_t metric);
 /* ed.cuh:61-69 finalize: squared z-normalised ED of windows 0..count-1
- * (the reference stores 2M-shifted values, ed.cuh:68; this stores ED^2). */
+ * (the reference stores 2M-shifted values, ed.cuh:68; this stores ED^2),
+ * summed in FP64: the exact direct-form distance for the statistics given,
+ * rounded once to FP32. */
 int mdtw_ed(mdtw_engine* e, const float* d_series, const float* d_avg,
             const float* d_std, float* d_out, int64_t count);
 

@@ -257,9 +257,10 @@ inline mdtw_compat::match prune_cdtw(value_t* Subject, value_t* AvgS, value_t* S
 
 // ed.cuh:71-153 calculate_ed: best z-normalised Euclidean match of the query
 // (set_query) in Subject[0..N).  The reference computes in FP64 through
-// cuFFT; this searches the FP32 stream (Subject float) directly for
-// M <= 2048 and through the FP64 FFT form above that (MDTW_ALGO_ED_FFT,
-// faster for long queries), with the window statistics computed here (StdS
+// cuFFT; this searches the FP32 stream (Subject float) directly (FP64 sums)
+// for M <= 256 and through the FP64 FFT form above that (MDTW_ALGO_ED_FFT,
+// faster from M = 384 on: 2.4 vs 2.9 ms for 2^27 samples, flat in M), with
+// the window statistics computed here (StdS
 // is accepted for signature parity and not read).  Prints the reference's
 // "distance:" / "location:" lines (ed.cuh:149-150) and returns its
 // bsf = ED^2 - 2M.
@@ -269,7 +270,7 @@ inline double calculate_ed(const value_t* const Subject, const value_t* const St
     (void)StdS;
     int64_t pos = -1;
     float dist = INFINITY;
-    const int32_t algo = M > 2048 ? MDTW_ALGO_ED_FFT : MDTW_ALGO_ED;
+    const int32_t algo = M > 256 ? MDTW_ALGO_ED_FFT : MDTW_ALGO_ED;
     mdtw_compat::check(mdtw_search(mdtw_compat::on_stream(stream), Subject,
                                    static_cast<int64_t>(N), nullptr, nullptr, 0, algo,
                                    MDTW_METRIC_L2, MDTW_STATS_FP64, 1, 0, &pos, &dist, nullptr),
