@@ -170,6 +170,27 @@ def timed(torch, fn, reps=1):
     return r, (time.perf_counter() - t0) / reps
 
 
+def ed_vendor_fft(torch, ds, zq):
+    """ED of every window in the reference's form (ed.cuh:71-153) with
+    torch.fft (rocFFT on this platform), FP64: a comparison for the
+    hand-written transform, not part of the library.  Returns (argmin, ED^2)."""
+    x = ds.double()
+    n, m = x.numel(), len(zq)
+    q = torch.zeros(n, dtype=torch.float64, device=ds.device)
+    q[:m] = torch.from_numpy(np.asarray(zq, dtype=np.float64)).to(ds.device)
+    corr = torch.fft.irfft(torch.fft.rfft(x) * torch.conj(torch.fft.rfft(q)), n=n)[:n - m + 1]
+    c = x[0]
+    p1 = torch.cumsum(torch.cat([x.new_zeros(1), x - c]), 0)
+    p2 = torch.cumsum(torch.cat([x.new_zeros(1), (x - c) ** 2]), 0)
+    s1 = (p1[m:] - p1[:-m]) / m
+    var = (p2[m:] - p2[:-m]) / m - s1 * s1
+    sig = torch.where(var > 0, var.clamp(min=0).sqrt(), torch.ones_like(var))
+    mu = s1 + c
+    ed2 = 2 * m - 2 * (corr - mu * float(np.sum(np.asarray(zq, np.float64)))) / sig
+    i = int(torch.argmin(ed2))
+    return i, float(ed2[i])
+
+
 def section_c2(eng, torch, args):
     from cudadtw_amd import ALGO_ED, ALGO_ED_FFT, synth, znormalize
 
@@ -316,13 +337,22 @@ def section_c2(eng, torch, args):
         eng.set_query(zz, 0)
         rd, td = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED), reps=3)
         rf, tf = timed(torch, lambda: eng.search(ds, 0, algo=ALGO_ED_FFT), reps=3)
+        # comparison only: the reference's own form through the vendor FFT
+        # (torch.fft is rocFFT here), ed.cuh:71-153 -- one FP64 transform of
+        # the whole stream and of the zero-padded query, the product, the
+        # inverse, FP64 window statistics from FP64 prefix sums, then
+        # ED^2 = 2M - 2 (corr - mu sum q) / sigma and the argmin
+        (loc_v, _), tv = timed(torch, lambda: ed_vendor_fft(torch, ds, zz), reps=2)
         Lq = n - mq + 1
         ed_forms[str(mq)] = {"direct_ms": round(td * 1e3, 3), "fft_ms": round(tf * 1e3, 3),
                              "fft_windows_per_s": round(Lq / tf, 1),
                              "direct_location": rd.location, "fft_location": rf.location,
-                             "same_location": rd.location == rf.location}
+                             "same_location": rd.location == rf.location,
+                             "rocfft_reference_form_ms": round(tv * 1e3, 3),
+                             "rocfft_reference_form_location": loc_v}
         log(f"[c2] ED |Q|={mq}: direct {td*1e3:.2f} ms (location {rd.location}), FFT "
-            f"{tf*1e3:.2f} ms (location {rf.location})")
+            f"{tf*1e3:.2f} ms (location {rf.location}), the reference's whole-stream form "
+            f"through rocFFT {tv*1e3:.2f} ms (location {loc_v})")
     eng.set_query(zq, w)
 
     # The same search fed from host memory: one-shot after a host-to-device
