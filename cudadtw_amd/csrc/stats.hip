@@ -82,13 +82,14 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // coalesced loads, lane l takes the kStatsR consecutive windows l*R .. l*R+R-1,
 // the transitions before them are a wave exclusive scan, and the results go
 // back through LDS for coalesced stores.  LDS index i is stored at
-// i + i/16, so lane l's consecutive elements (17l + v) fall in distinct banks.
+// i + i/R, so lane l's consecutive elements ((R+1)l + v) fall in distinct
+// banks.
 constexpr int kStatsR = 16;
 constexpr int kStatsTW = kWave * kStatsR;
 constexpr int kStatsWaves = 4;
-constexpr int kStatsLds = kStatsTW + kStatsTW / 16;
+constexpr int kStatsLds = kStatsTW + kStatsTW / kStatsR;
 
-__device__ __forceinline__ int stats_skew(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int stats_skew(int i) { return i + i / kStatsR; }
 
 // Wave-wide inclusive scan of a double through DPP (GFX9 family): row_shr
 // 1/2/4/8 inside 16-lane rows, then row_bcast:15 and row_bcast:31 carry row
