@@ -498,12 +498,17 @@ def section_c4(eng, torch, args):
     t_batch = t_plain = t_tune = 0.0
     nb = 0
     same = True
+    forms = {}  # DP formulation the tuner picked, per query
     for g0 in range(0, len(zqs), G):
         gz, gb = zqs[g0:g0 + G], bs[g0:g0 + G]
         batch = eng.batch(gz, gb, pipelines=4)
         t0 = time.perf_counter()
         nb += batch.tune(ds, k=k, cell_budget=1 << 38)
         t_tune += time.perf_counter() - t0
+        for i, (z, w) in enumerate(zip(gz, gb)):
+            sh = batch.shape(i)
+            key = f"{'W<=112' if w <= 112 else 'W113-240' if w <= 240 else 'W>240'}:{sh['dp_form']}"
+            forms[key] = forms.get(key, 0) + 1
         rb = batch.search(ds, k=k)  # captures the graph
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -538,7 +543,7 @@ def section_c4(eng, torch, args):
             "wide_band_queries": int(np.sum(bs > 112)),
             "batch_s": round(t_batch, 4), "plain_s": round(t_plain, 4),
             "speedup": round(t_plain / t_batch, 3), "identical": same,
-            "tune_s": round(t_tune, 2), "buckets_tuned": nb,
+            "tune_s": round(t_tune, 2), "buckets_tuned": nb, "tuned_dp_forms": forms,
             "launch_bound_case": short,
             "windows_per_s_batch": round(float(Ls.sum()) / t_batch, 1),
             "effective_gcups_batch": round(cellsum / t_batch / 1e9, 2)}

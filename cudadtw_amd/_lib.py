@@ -98,6 +98,8 @@ SIGNATURES = {
                                     C.POINTER(_P)]),
     "mdtw_batch_destroy": (C.c_int, [_P]),
     "mdtw_batch_tune": (C.c_int, [_P, _P, _I64, _I32, _I32, _I64, C.POINTER(_I32)]),
+    "mdtw_batch_shape": (C.c_int, [_P, _I32, C.POINTER(_I32)]),
+    "mdtw_batch_set_shape": (C.c_int, [_P, _I32, C.POINTER(_I32)]),
     "mdtw_batch_search": (C.c_int, [_P, _P, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64),
                                     _FP, C.POINTER(SearchInfo), _FP]),
     "mdtw_batch_backward": (C.c_int, [_P, _P, _I64, _I32, _I32, C.POINTER(_I64), _FP, _I32, _FP,

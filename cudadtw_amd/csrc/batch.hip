@@ -13,11 +13,15 @@
 // pipeline), so the pipelines end together; their kernels overlap on the
 // device, which fills the tails of one query's kernels with another's work.
 //
-// Launch shapes (persistent DP grid, anti-diagonal DP grid, cascade grid) are
-// picked per (|Q|, band) bucket by mdtw_batch_tune: each candidate shape runs
-// the search of the bucket's median-cost query on a prefix of the stream
-// and the fastest is kept.  Shapes change only how work is spread over the
-// device, never the result.
+// Launch shapes are picked per (|Q|, band) bucket by mdtw_batch_tune: the
+// persistent DP grid (windows per workgroup pass times workgroups), the
+// anti-diagonal DP grid and the cascade grid, and then the DP's formulation
+// where more than one applies (DpForm: register band with or without the
+// cumulative LB_Keogh bound, or the anti-diagonal kernel with the band's
+// diagonals in LDS for W = 113..240).  Each candidate runs the search of the
+// bucket's median-cost query on a prefix of the stream and the fastest is
+// kept.  Shapes change only how work is spread over the device, never the
+// result.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -39,11 +43,12 @@ int64_t band_cells(int32_t m, int32_t w) {
 }
 
 // Bucket of a query for launch-shape tuning: power-of-two length class and
-// whether the band fits the register-band kernel.
+// band class (one lane per window with the band in registers; the LDS-tail
+// and two-wave split bands; the anti-diagonal kernel only).
 int shape_bucket(int32_t m, int32_t w) {
     int lg = 0;
     while ((1 << (lg + 1)) <= m) ++lg;
-    return lg * 2 + (w > kBandMaxW ? 1 : 0);
+    return lg * 3 + (w <= kBandMaxW ? 0 : w <= kBandWideMaxW ? 1 : 2);
 }
 
 int hip_ok(hipError_t err) { return hip_status(err); }
@@ -474,6 +479,31 @@ int mdtw_batch_destroy(mdtw_batch* b) {
     return MDTW_OK;
 }
 
+int mdtw_batch_shape(const mdtw_batch* b, int32_t i, int32_t* h_shape) {
+    if (!b || !h_shape || i < 0 || i >= b->nq) return MDTW_ERR_INVALID_ARGUMENT;
+    const LaunchShape& s = b->shape_of(i);
+    h_shape[0] = s.dp_blocks_per_cu;
+    h_shape[1] = s.wide_waves_per_cu;
+    h_shape[2] = s.cascade_blocks_per_cu;
+    h_shape[3] = s.dp_form;
+    return MDTW_OK;
+}
+
+int mdtw_batch_set_shape(mdtw_batch* b, int32_t i, const int32_t* h_shape) {
+    if (!b || !h_shape || i < 0 || i >= b->nq) return MDTW_ERR_INVALID_ARGUMENT;
+    if (h_shape[0] < 1 || h_shape[0] > 64 || h_shape[1] < 1 || h_shape[1] > 256 ||
+        h_shape[2] < 1 || h_shape[2] > 256 || h_shape[3] < kDpFormAuto || h_shape[3] > kDpFormWide)
+        return MDTW_ERR_INVALID_ARGUMENT;
+    LaunchShape s;
+    s.dp_blocks_per_cu = h_shape[0];
+    s.wide_waves_per_cu = h_shape[1];
+    s.cascade_blocks_per_cu = h_shape[2];
+    s.dp_form = h_shape[3];
+    b->shapes[shape_bucket(b->m[i], b->w[i])] = s;
+    b->shapes_version++;
+    return MDTW_OK;
+}
+
 int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n, int32_t metric, int32_t k,
                     int64_t cell_budget, int32_t* h_buckets_tuned) {
     int rc = check_common(b, d_series, n, metric, k);
@@ -509,16 +539,38 @@ int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n, int32_t met
             }
         float best_ms = INFINITY;
         LaunchShape best;
-        for (const LaunchShape& s : cands) {
+        auto trial = [&](const LaunchShape& s) {
             float ms = 0.0f;
-            rc = run_one(b, i, d_series, ns, metric, k, ex[i], s, nullptr, nullptr, nullptr, &ms);
-            if (rc == MDTW_OK)
-                rc = run_one(b, i, d_series, ns, metric, k, ex[i], s, nullptr, nullptr, nullptr,
-                             &ms);
-            if (rc != MDTW_OK) return rc;
-            if (ms < best_ms) {
+            int r = run_one(b, i, d_series, ns, metric, k, ex[i], s, nullptr, nullptr, nullptr, &ms);
+            if (r == MDTW_OK)
+                r = run_one(b, i, d_series, ns, metric, k, ex[i], s, nullptr, nullptr, nullptr, &ms);
+            if (r == MDTW_OK && ms < best_ms) {
                 best_ms = ms;
                 best = s;
+            }
+            return r;
+        };
+        for (const LaunchShape& s : cands) {
+            rc = trial(s);
+            if (rc != MDTW_OK) return rc;
+        }
+        // Then the DP's formulation, on the grids just chosen: without the
+        // cumulative bound (one-lane register bands, where it is used), and
+        // for the LDS-tail / split bands the anti-diagonal kernel.
+        const LaunchShape grid_best = best;
+        if (b->w[i] < kBandPairMinW) {
+            LaunchShape s = grid_best;
+            s.dp_form = kDpFormNoLb;
+            rc = trial(s);
+            if (rc != MDTW_OK) return rc;
+        }
+        if (wide && b->w[i] <= kBandWideMaxW) {
+            for (int32_t ww : {16, 32, 64}) {
+                LaunchShape s = grid_best;
+                s.dp_form = kDpFormWide;
+                s.wide_waves_per_cu = ww;
+                rc = trial(s);
+                if (rc != MDTW_OK) return rc;
             }
         }
         b->shapes[kv.first] = best;

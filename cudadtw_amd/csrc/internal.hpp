@@ -207,10 +207,25 @@ struct BandScalars {
 
 // Launch shapes of the search kernels; the batch autotuner picks them per
 // query-length / band bucket, plain searches use the defaults.
+// DP formulation of a search (LaunchShape::dp_form).  Every form computes the
+// same bits for every window it completes and abandons only above the bound,
+// so the form changes the time, never the result:
+//   kDpFormAuto:  the register-band kernel of the band (one lane per window,
+//                 its band in registers, an LDS tail or split over two waves
+//                 as W requires), with the cumulative LB_Keogh abandon where
+//                 its register cost leaves the residency; the anti-diagonal
+//                 kernel above W = 240;
+//   kDpFormNoLb:  the same kernels without the cumulative LB_Keogh bound
+//                 (column-minimum abandon only; fewer registers);
+//   kDpFormWide:  the anti-diagonal kernel (the band across a wave, diagonals
+//                 through LDS) for any band.
+enum DpForm : int32_t { kDpFormAuto = 0, kDpFormNoLb = 1, kDpFormWide = 2 };
+
 struct LaunchShape {
     int32_t dp_blocks_per_cu = 8;         // persistent register-band DP (4 waves each)
     int32_t wide_waves_per_cu = 32;       // persistent anti-diagonal DP (1 wave each)
     int32_t cascade_blocks_per_cu = 64;   // cap of the cascade grid (4 waves each)
+    int32_t dp_form = kDpFormAuto;        // DpForm
 };
 
 // Everything one search needs on the device (enqueue_search).

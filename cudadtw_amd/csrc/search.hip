@@ -160,7 +160,8 @@ namespace {
 // One DP-style launch for the selected algorithm.
 int run_algo(const SearchPlan& p, const BandArgs& a) {
     if (p.algo == MDTW_ALGO_CDTW)
-        return launch_dp(p.st, p.cus, a, p.q, p.w, p.metric, false, p.shape);
+        return launch_dp(p.st, p.cus, a, p.q, p.w, p.metric, p.shape.dp_form == kDpFormWide,
+                         p.shape);
     BandArgs b = a;
     b.qx = p.q;
     b.ed = p.ed;
@@ -272,7 +273,7 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
             if (rc == MDTW_OK) {
                 BandArgs da = a;
                 da.idx = cand;
-                da.lbt = cand_lb;
+                da.lbt = p.shape.dp_form == kDpFormNoLb ? nullptr : cand_lb;
                 da.lo = p.lo;
                 da.up = p.up;
                 da.idx_count_from_ctrl = 1;

@@ -277,13 +277,28 @@ int mdtw_batch_create(mdtw_engine* e, const float* h_zqueries,
                       const int32_t* h_lengths, const int32_t* h_bands,
                       int32_t nq, int32_t pipelines, mdtw_batch** out);
 int mdtw_batch_destroy(mdtw_batch* b);
-/* Picks launch shapes (persistent DP grid, anti-diagonal DP grid, cascade
- * grid) per bucket of queries (power-of-two length class x register-band or
- * wide band) by timing the bucket's median query on a stream prefix of
- * about cell_budget band cells (<= 0: 2^39).  Affects speed only. */
+/* Picks launch shapes per bucket of queries (power-of-two length class x
+ * band class: register band W <= 112, LDS-tail / two-wave split band
+ * W <= 240, anti-diagonal above) by timing the bucket's median query on a
+ * stream prefix of about cell_budget band cells (<= 0: 2^39): the persistent
+ * DP grid, the anti-diagonal DP grid and the cascade grid, then the DP's
+ * formulation where more than one applies (MDTW_DP_FORM_*).  Affects speed
+ * only. */
 int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n,
                     int32_t metric, int32_t k, int64_t cell_budget,
                     int32_t* h_buckets_tuned);
+#define MDTW_DP_FORM_AUTO 0  /* register band (+ cumulative LB_Keogh bound where it fits) */
+#define MDTW_DP_FORM_NOLB 1  /* register band, column-minimum abandon only             */
+#define MDTW_DP_FORM_WIDE 2  /* anti-diagonal kernel, the band's diagonals in LDS      */
+/* No reference counterpart: the shape query i's searches use (default or
+ * tuned): h_shape[0..4) = persistent DP workgroups per CU, anti-diagonal
+ * waves per CU, cascade workgroups per CU, MDTW_DP_FORM_*. */
+int mdtw_batch_shape(const mdtw_batch* b, int32_t i, int32_t* h_shape);
+/* No reference counterpart: sets the shape of query i's bucket (h_shape as
+ * above: 1..64 DP workgroups per CU, 1..256 anti-diagonal waves per CU,
+ * 1..256 cascade workgroups per CU, an MDTW_DP_FORM_*), as a tuning result
+ * kept from an earlier run would.  Affects speed only. */
+int mdtw_batch_set_shape(mdtw_batch* b, int32_t i, const int32_t* h_shape);
 /* Best (k = 1) or top-k cDTW matches of every query (mdtw_search semantics,
  * window statistics computed per query length in FP64).  h_excl: nq
  * exclusion zones, or NULL for M/2 each.  Outputs: h_pos/h_dist nq*k

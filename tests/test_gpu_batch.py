@@ -93,6 +93,34 @@ def test_batch_tuned_shapes_keep_results(engine):
     b.close()
 
 
+def test_batch_every_dp_form_gives_the_same_results(engine):
+    # The tuner may run a bucket's DP as the register band with or without
+    # the cumulative LB_Keogh bound, or as the anti-diagonal kernel; every
+    # form must return the same positions and distance bits.  Queries cover
+    # the register-band, LDS-tail / split and anti-diagonal band classes.
+    s = synth.ecg_like(1 << 15, 9)
+    ds = dev(s)
+    specs = [(96, 10), (128, 25), (300, 60), (320, 118), (400, 130), (600, 200), (700, 260)]
+    qs, bands = [], []
+    for j, (m, w) in enumerate(specs):
+        q, _ = synth.cut_query(s, m, 40 + j, noise=0.1)
+        qs.append(O.znormalize_f32(q))
+        bands.append(w)
+    b = engine.batch(qs, bands, pipelines=2)
+    r0 = b.search(ds, k=3)
+    for form in ("no_lb", "wide"):
+        for i in range(len(specs)):
+            b.set_shape(i, dp_blocks_per_cu=4, wide_waves_per_cu=16, cascade_blocks_per_cu=16,
+                        dp_form=form)
+            assert b.shape(i)["dp_form"] == form
+        r = b.search(ds, k=3)
+        np.testing.assert_array_equal(r.positions, r0.positions)
+        np.testing.assert_array_equal(r.distances, r0.distances)
+    with pytest.raises(Exception):
+        b.set_shape(0, dp_blocks_per_cu=0)
+    b.close()
+
+
 def test_batch_argument_errors(engine):
     from cudadtw_amd._lib import MdtwError
     q = O.znormalize_f32(synth.random_walk(32, 1))
