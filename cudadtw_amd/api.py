@@ -530,13 +530,13 @@ class Batch:
               "mdtw_batch_shape")
         return {"dp_blocks_per_cu": int(out[0]), "wide_waves_per_cu": int(out[1]),
                 "cascade_blocks_per_cu": int(out[2]),
-                "dp_form": ("auto", "no_lb", "wide")[int(out[3])]}
+                "dp_form": ("auto", "lb", "wide")[int(out[3])]}
 
     def set_shape(self, i: int, dp_blocks_per_cu: int = 8, wide_waves_per_cu: int = 32,
                   cascade_blocks_per_cu: int = 64, dp_form: str = "auto") -> None:
         """Fix the launch shape of query i's bucket (mdtw_batch_set_shape)."""
         v = np.array([dp_blocks_per_cu, wide_waves_per_cu, cascade_blocks_per_cu,
-                      ("auto", "no_lb", "wide").index(dp_form)], dtype=np.int32)
+                      ("auto", "lb", "wide").index(dp_form)], dtype=np.int32)
         check(self.engine.lib.mdtw_batch_set_shape(self.h, int(i),
                                                    v.ctypes.data_as(C.POINTER(C.c_int32))),
               "mdtw_batch_set_shape")

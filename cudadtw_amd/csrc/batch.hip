@@ -554,13 +554,13 @@ int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n, int32_t met
             rc = trial(s);
             if (rc != MDTW_OK) return rc;
         }
-        // Then the DP's formulation, on the grids just chosen: without the
-        // cumulative bound (one-lane register bands, where it is used), and
-        // for the LDS-tail / split bands the anti-diagonal kernel.
+        // Then the DP's formulation, on the grids just chosen: with the
+        // cumulative LB_Keogh bound (one-lane register bands), and for the
+        // LDS-tail / split bands the anti-diagonal kernel.
         const LaunchShape grid_best = best;
         if (b->w[i] < kBandPairMinW) {
             LaunchShape s = grid_best;
-            s.dp_form = kDpFormNoLb;
+            s.dp_form = kDpFormLb;
             rc = trial(s);
             if (rc != MDTW_OK) return rc;
         }

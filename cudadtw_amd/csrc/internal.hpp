@@ -212,14 +212,16 @@ struct BandScalars {
 // so the form changes the time, never the result:
 //   kDpFormAuto:  the register-band kernel of the band (one lane per window,
 //                 its band in registers, an LDS tail or split over two waves
-//                 as W requires), with the cumulative LB_Keogh abandon where
-//                 its register cost leaves the residency; the anti-diagonal
-//                 kernel above W = 240;
-//   kDpFormNoLb:  the same kernels without the cumulative LB_Keogh bound
-//                 (column-minimum abandon only; fewer registers);
+//                 as W requires), abandoning on the column minimum; the
+//                 anti-diagonal kernel above W = 240;
+//   kDpFormLb:    the same kernels, one lane per window also abandoning on
+//                 the column minimum plus the LB_Keogh terms of the columns
+//                 still to come (where its registers leave the residency);
+//                 measured slower than kDpFormAuto on every plain-search
+//                 workload here (1-20%), kept for the tuner;
 //   kDpFormWide:  the anti-diagonal kernel (the band across a wave, diagonals
 //                 through LDS) for any band.
-enum DpForm : int32_t { kDpFormAuto = 0, kDpFormNoLb = 1, kDpFormWide = 2 };
+enum DpForm : int32_t { kDpFormAuto = 0, kDpFormLb = 1, kDpFormWide = 2 };
 
 struct LaunchShape {
     int32_t dp_blocks_per_cu = 8;         // persistent register-band DP (4 waves each)

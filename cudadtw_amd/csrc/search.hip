@@ -268,12 +268,13 @@ int enqueue_search(const SearchPlan& p, SearchBufs* bufs) {
         } else if (m >= 11) {
             // The candidate count and work cursor are 0 here: the search
             // reset cleared them, and between rounds the tighten does.
+            const bool lbf = p.shape.dp_form == kDpFormLb;
             rc = launch_cascade(st, p.s, avg, sd, p.q, p.lo, p.up, m, r.base, r.count, p.metric,
-                                !topk, p.ctrl, cand, cand_lb, cascade_cap, p.cus);
+                                !topk, p.ctrl, cand, lbf ? cand_lb : nullptr, cascade_cap, p.cus);
             if (rc == MDTW_OK) {
                 BandArgs da = a;
                 da.idx = cand;
-                da.lbt = p.shape.dp_form == kDpFormNoLb ? nullptr : cand_lb;
+                da.lbt = lbf ? cand_lb : nullptr;
                 da.lo = p.lo;
                 da.up = p.up;
                 da.idx_count_from_ctrl = 1;

@@ -287,9 +287,9 @@ int mdtw_batch_destroy(mdtw_batch* b);
 int mdtw_batch_tune(mdtw_batch* b, const float* d_series, int64_t n,
                     int32_t metric, int32_t k, int64_t cell_budget,
                     int32_t* h_buckets_tuned);
-#define MDTW_DP_FORM_AUTO 0  /* register band (+ cumulative LB_Keogh bound where it fits) */
-#define MDTW_DP_FORM_NOLB 1  /* register band, column-minimum abandon only             */
-#define MDTW_DP_FORM_WIDE 2  /* anti-diagonal kernel, the band's diagonals in LDS      */
+#define MDTW_DP_FORM_AUTO 0  /* register band, column-minimum abandon (anti-diagonal above W = 240) */
+#define MDTW_DP_FORM_LB   1  /* register band, also the cumulative LB_Keogh bound (one lane per window) */
+#define MDTW_DP_FORM_WIDE 2  /* anti-diagonal kernel, the band's diagonals in LDS */
 /* No reference counterpart: the shape query i's searches use (default or
  * tuned): h_shape[0..4) = persistent DP workgroups per CU, anti-diagonal
  * waves per CU, cascade workgroups per CU, MDTW_DP_FORM_*. */

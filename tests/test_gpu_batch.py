@@ -108,7 +108,7 @@ def test_batch_every_dp_form_gives_the_same_results(engine):
         bands.append(w)
     b = engine.batch(qs, bands, pipelines=2)
     r0 = b.search(ds, k=3)
-    for form in ("no_lb", "wide"):
+    for form in ("lb", "wide"):
         for i in range(len(specs)):
             b.set_shape(i, dp_blocks_per_cu=4, wide_waves_per_cu=16, cascade_blocks_per_cu=16,
                         dp_form=form)
