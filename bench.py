@@ -290,6 +290,15 @@ def section_c2(eng, torch, args):
                                      "frac": round(ed_tflops / PEAK_FP64_TFLOPS, 4),
                                      "flops_per_window": 2 * m,
                                      "valu_issue_frac": round(issue, 3)}
+    # its HBM bytes per launch, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # at this shape (profiles/ed_traffic.json), scaled to the windows
+    epath = os.path.join(ROOT, "profiles", "ed_traffic.json")
+    if os.path.exists(epath):
+        with open(epath) as f:
+            et = json.load(f)
+        if et.get("m") == m:
+            mem["ed_profile"]["roofline"]["traffic"] = round(et["bytes_per_window"] * L)
+            mem["ed_profile"]["roofline"]["traffic_unit"] = "bytes per launch (HBM, rocprofv3 PMC)"
     # a plain device copy of the stream (torch's copy kernel): the HBM rate a
     # 1 read : 1 write pattern reaches here, beside the kernels above
     dst = torch.empty_like(ds)
