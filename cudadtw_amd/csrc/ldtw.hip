@@ -103,7 +103,17 @@ __device__ __forceinline__ float ldtw_window(const float* __restrict__ x, float 
             qi = qi1;
             qi1 = i + 1 < M ? q[i + 1] : 0.0f;
         }
-        if (adv_j && j + 1 < M) {
+        if constexpr (FASTZ) {
+            // stream advance as selects: the next value is loaded and
+            // normalised every step (four operations through band_z_fast),
+            // which costs less than the divergent branch's exec-mask work
+            const bool aj = adv_j && j + 1 < M;
+            const float zn = ldtw_z<FASTZ>(x[j + 2 < M ? j + 2 : M - 1], mu, sg, y);
+            const float zn1 = j + 2 < M ? zn : 0.0f;
+            zj = aj ? zj1 : zj;
+            zj1 = aj ? zn1 : zj1;
+            j = aj ? j + 1 : j;
+        } else if (adv_j && j + 1 < M) {
             ++j;
             zj = zj1;
             zj1 = j + 1 < M ? ldtw_z<FASTZ>(x[j + 1], mu, sg, y) : 0.0f;
