@@ -282,7 +282,8 @@ def test_ed_vs_fp64(engine, m):
 @pytest.mark.parametrize("n,m,kind", [(300, 256, "ecg"), (1000, 256, "ecg"), (1 << 15, 16, "ecg"),
                                         (40_000, 1000, "ecg"), (7777, 333, "ecg"),
                                         (70_000, 2048, "ecg"), (50_000, 300, "metal7000"),
-                                        (20_000, 128, "walk")])
+                                        (20_000, 128, "walk"), (5_003, 12, "walk"),
+                                        (4_099, 3, "walk")])
 def test_ed_exact_over_its_statistics(engine, n, m, kind):
     # The ED kernels sum in FP64 (centred query, group-shifted samples), so a
     # window's distance is the exact direct form over the FP32 statistics it
