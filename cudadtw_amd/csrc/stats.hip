@@ -90,6 +90,12 @@ constexpr int kStatsWaves = 4;
 constexpr int kStatsLds = kStatsTW + kStatsTW / kStatsR;
 
 __device__ __forceinline__ int stats_skew(int i) { return i + i / kStatsR; }
+// stats_skew(v * kWave + lane) written as one lane base plus a constant per
+// v, so unrolled accesses share one address register (immediate offsets).
+static_assert(kWave % kStatsR == 0, "a wave row must hold whole skew groups");
+__device__ __forceinline__ int stats_slot(int v, int lane) {
+    return v * (kWave + kWave / kStatsR) + lane + lane / kStatsR;
+}
 
 // Wave-wide inclusive scan of a double through DPP (GFX9 family): row_shr
 // 1/2/4/8 inside 16-lane rows, then row_bcast:15 and row_bcast:31 carry row
@@ -133,73 +139,84 @@ __device__ __forceinline__ void stats_wave_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One tile [w0, w0 + nw) of windows by one wave (w0 a multiple of kStatsTW):
-// leaves mean and sigma of window w0 + i in xa / xb at stats_skew(i).  The
-// whole-stream kernel and the window-list kernel below both run exactly this,
-// so a window's statistics do not depend on which of them computed it.
-// T = double (the FFT ED's statistics): the same sums, leaving the FP64 mean
-// and the FP64 variance (0 for a constant window) instead.  S is the sample
-// type (FP32 streams, or FP64 ones for the FP64 ED; T must hold S exactly).
-// RAW (MDTW_STATS_NONE): mean 0 and sigma 1, which make (x - mean) / sigma
-// == x exactly in every kernel -- only the non-finite rule is computed.
-// A window holding a non-finite sample gets NaN mean and sigma (variance).
-template <class T = float, class S = float, bool RAW = false>
-__device__ __forceinline__ int stats_tile(const S* __restrict__ s, int64_t L, int32_t m,
-                                          int64_t w0, T* xa, T* xb) {
-    static_assert(sizeof(T) >= sizeof(S), "the LDS tile must hold samples exactly");
+// Loads of one tile's samples into registers: x[w0 + i] (leaving sample of
+// window i) and x[w0 + m + i] (entering), i = v * 64 + lane, clamped to the
+// last stream sample.  Issued together; nothing waits on them until
+// stats_stage, so a sweep can have the next tile's loads in flight while it
+// computes the current one.
+template <class S>
+__device__ __forceinline__ void stats_load(const S* __restrict__ s, int64_t L, int32_t m,
+                                           int64_t w0, S (&va)[kStatsR], S (&vb)[kStatsR]) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int nw = L - w0 < kStatsTW ? static_cast<int>(L - w0) : kStatsTW;
     const int64_t last = L - 1 + m - 1;  // last stream sample
-    const S* x = s + w0;
-    const double c = isfinite(x[0]) ? static_cast<double>(x[0]) : 0.0;
-
-    // Stage x[w0 + i] (leaving sample of window i) and x[w0 + m + i] (entering);
-    // every load of the tile is issued before any is used.
-    S va[kStatsR], vb[kStatsR];
+    // Two wave-uniform bases and one lane offset, so the loads hold no
+    // per-load 64-bit address.
+    const S* __restrict__ pa = s + w0;
+    const S* __restrict__ pb = pa + m;
+    if (w0 + m + kStatsTW - 1 <= last) {  // whole tile inside the stream
+        const S* __restrict__ qa = pa + lane;
+        const S* __restrict__ qb = pb + lane;
 #pragma unroll
-    for (int v = 0; v < kStatsR; ++v) {
-        const int i = v * kWave + lane;
-        const int64_t ga = w0 + i, gb = w0 + m + i;
-        va[v] = s[ga < last ? ga : last];
-        vb[v] = s[gb < last ? gb : last];
+        for (int v = 0; v < kStatsR; ++v) {  // immediate offsets from qa / qb
+            va[v] = qa[v * kWave];
+            vb[v] = qb[v * kWave];
+        }
+        return;
     }
+    // The stream's last tile: sample indices clamped to last, written as
+    // v * 64 + min(lane, bound_v) with a wave-uniform bound_v (here
+    // last - w0 < m + kStatsTW fits an int; an index min(lane, negative) + ...
+    // still lands on last itself).
+    const int la = static_cast<int>(last - w0);
 #pragma unroll
     for (int v = 0; v < kStatsR; ++v) {
-        const int i = v * kWave + lane;
-        xa[stats_skew(i)] = va[v];
-        xb[stats_skew(i)] = vb[v];
+        const int ca = la - v * kWave, cb = la - m - v * kWave;
+        va[v] = pa[v * kWave + (lane < ca ? lane : ca)];
+        vb[v] = pb[v * kWave + (lane < cb ? lane : cb)];
+    }
+}
+
+// Stages the loaded samples in LDS; returns (wave-uniform) whether all of
+// them are finite.
+template <class T, class S>
+__device__ __forceinline__ bool stats_stage(const S (&va)[kStatsR], const S (&vb)[kStatsR], T* xa,
+                                            T* xb) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int fin = 1;
+#pragma unroll
+    for (int v = 0; v < kStatsR; ++v) {
+        xa[stats_slot(v, lane)] = va[v];
+        xb[stats_slot(v, lane)] = vb[v];
+        // bitwise, not short-circuit: no branch per sample
+        fin &= static_cast<int>(isfinite(va[v])) & static_cast<int>(isfinite(vb[v]));
     }
     stats_wave_fence();
+    return __all(fin);
+}
 
-    // First window of the tile: x[w0 .. w0 + m), from the staged tile where
-    // it lies inside it (x[w0 + k] = xa[k] for k < kStatsTW).
-    double a1 = 0.0, a2 = 0.0, b1 = 0.0, f1 = 0.0;
-    for (int k = lane; k < m; k += kWave) {
-        const S xk = k < kStatsTW ? static_cast<S>(xa[stats_skew(k)]) : x[k];
-        const bool fin = isfinite(xk);
-        const double d = fin ? static_cast<double>(xk) - c : 0.0;
-        a1 += d;
-        b1 += fabs(d);
-        a2 = fma(d, d, a2);
-        f1 += fin ? 0.0 : 1.0;
-    }
-    const double S1 = wave_sum_f64(a1);
-    const double S2 = wave_sum_f64(a2);
-    const double B1 = wave_sum_f64(b1);
-    const double F1 = wave_sum_f64(f1);  // non-finite samples of the first window
-
+// The windows of a staged tile from the first window's sums (S1, S2, B1 of
+// x - c, F1 non-finite samples): pass 1 totals each lane's transitions, a
+// wave scan offsets them, pass 2 walks the lane's windows.  CHECK keeps the
+// non-finite bookkeeping (needed only when the tile holds such a sample).
+template <class T, class S, bool RAW, bool CHECK>
+__device__ __forceinline__ void stats_windows(int32_t m, int nw, double c, double S1, double S2,
+                                              double B1, double F1, T* xa, T* xb) {
+    const int lane = threadIdx.x & (kWave - 1);
     // Pass 1: this lane's transition totals (window r -> r + 1).
     const int r0 = lane * kStatsR;
     // A non-finite sample enters the sums as 0 and the count as 1.
     auto shifted = [c](T v, double& fcount) {
+        if constexpr (!CHECK) return static_cast<double>(v) - c;
         const bool fin = isfinite(v);
         fcount += fin ? 0.0 : 1.0;
         return fin ? static_cast<double>(v) - c : 0.0;
     };
     double t1 = 0.0, t2 = 0.0, u1 = 0.0, u2 = 0.0, tf = 0.0;
+    // r0 + v < nw as v < nv: one per-lane bound, compared with constants
+    const int nv = nw - r0;
     for (int v = 0; v < kStatsR; ++v) {
         const int r = r0 + v;
-        if (r + 1 < nw) {
+        if (v + 1 < nv) {
             double fo = 0.0, fn = 0.0;
             const double xo = shifted(xa[stats_skew(r)], fo);
             const double xn = shifted(xb[stats_skew(r)], fn);
@@ -207,21 +224,21 @@ __device__ __forceinline__ int stats_tile(const S* __restrict__ s, int64_t L, in
             t2 += xn * xn - xo * xo;
             u1 += fabs(xn) + fabs(xo);
             u2 += xn * xn + xo * xo;
-            tf += fn - fo;
+            if constexpr (CHECK) tf += fn - fo;
         }
     }
     double p1 = wave_excl_scan_f64(t1);
     double p2 = wave_excl_scan_f64(t2);
     double q1 = wave_excl_scan_f64(u1);
     double q2 = wave_excl_scan_f64(u2);
-    double pf = wave_excl_scan_f64(tf);
+    double pf = CHECK ? wave_excl_scan_f64(tf) : 0.0;
 
     // Pass 2: the lane's windows; results overwrite their own LDS slots
     // after the slot's samples are read.
     const double inv_m = 1.0 / static_cast<double>(m);
     for (int v = 0; v < kStatsR; ++v) {
         const int r = r0 + v;
-        if (r >= nw) break;
+        if (v >= nv) break;
         const double sum1 = S1 + p1;
         const double sum2 = S2 + p2;
         const double mean = sum1 * inv_m;
@@ -233,7 +250,8 @@ __device__ __forceinline__ int stats_tile(const S* __restrict__ s, int64_t L, in
         const double tol = 64.0 * DBL_EPSILON * (mag2 + 2.0 * fabs(mean) * mag1);
         if (var <= tol) var = 0.0;
         const int k = stats_skew(r);
-        const bool bad = F1 + pf != 0.0;  // the window holds a non-finite sample
+        // the window holds a non-finite sample
+        const bool bad = CHECK && F1 + pf != 0.0;
         double fo = 0.0, fn = 0.0;
         const double xo = shifted(xa[k], fo);
         const double xn = shifted(xb[k], fn);
@@ -254,10 +272,102 @@ __device__ __forceinline__ int stats_tile(const S* __restrict__ s, int64_t L, in
         p2 += xn * xn - xo * xo;
         q1 += fabs(xn) + fabs(xo);
         q2 += xn * xn + xo * xo;
-        pf += fn - fo;
+        if constexpr (CHECK) pf += fn - fo;
     }
+}
+
+// One tile [w0, w0 + nw) of windows by one wave (w0 a multiple of kStatsTW),
+// its samples already staged by stats_stage: leaves mean and sigma of window
+// w0 + i in xa / xb at stats_skew(i).  Every kernel below runs exactly this,
+// so a window's statistics do not depend on which of them computed it.
+// T = double (the FFT ED's statistics): the same sums, leaving the FP64 mean
+// and the FP64 variance (0 for a constant window) instead.  S is the sample
+// type (FP32 streams, or FP64 ones for the FP64 ED; T must hold S exactly).
+// RAW (MDTW_STATS_NONE): mean 0 and sigma 1, which make (x - mean) / sigma
+// == x exactly in every kernel -- only the non-finite rule is computed.
+// A window holding a non-finite sample gets NaN mean and sigma (variance).
+// Global memory is read only for a first window longer than the tile
+// (m > kStatsTW).
+template <class T = float, class S = float, bool RAW = false>
+__device__ __forceinline__ int stats_tile_staged(const S* __restrict__ s, int64_t L, int32_t m,
+                                                 int64_t w0, T* xa, T* xb, bool staged_finite) {
+    static_assert(sizeof(T) >= sizeof(S), "the LDS tile must hold samples exactly");
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nw = L - w0 < kStatsTW ? static_cast<int>(L - w0) : kStatsTW;
+    const S* x = s + w0;
+    const S x0 = static_cast<S>(xa[stats_skew(0)]);  // x[w0]
+    const double c = isfinite(x0) ? static_cast<double>(x0) : 0.0;
+
+    // First window of the tile: x[w0 .. w0 + m), from the staged tile where
+    // it lies inside it (x[w0 + k] = xa[k] for k < kStatsTW).
+    double a1 = 0.0, a2 = 0.0, b1 = 0.0, f1 = 0.0;
+    auto first = [&](S xk) {
+        const bool fin = isfinite(xk);
+        const double d = fin ? static_cast<double>(xk) - c : 0.0;
+        a1 += d;
+        b1 += fabs(d);
+        a2 = fma(d, d, a2);
+        f1 += fin ? 0.0 : 1.0;
+    };
+    const int mt = m < kStatsTW ? m : kStatsTW;
+    int k = lane;
+    for (; k < mt; k += kWave) first(static_cast<S>(xa[stats_skew(k)]));
+    for (; k < m; k += kWave) first(x[k]);
+    const double S1 = wave_sum_f64(a1);
+    const double S2 = wave_sum_f64(a2);
+    const double B1 = wave_sum_f64(b1);
+    const double F1 = wave_sum_f64(f1);  // non-finite samples of the first window
+
+    // Every sample a window of the tile reads is in the first window or
+    // among the staged ones: when all of those are finite (the usual case),
+    // the transitions skip the non-finite bookkeeping -- the same sums
+    // otherwise, so the same results.
+    if (staged_finite && F1 == 0.0)
+        stats_windows<T, S, RAW, false>(m, nw, c, S1, S2, B1, F1, xa, xb);
+    else
+        stats_windows<T, S, RAW, true>(m, nw, c, S1, S2, B1, F1, xa, xb);
     stats_wave_fence();
     return nw;
+}
+
+// Load, stage and compute one tile (the window-list kernel's form).
+template <class T = float, class S = float, bool RAW = false>
+__device__ __forceinline__ int stats_tile(const S* __restrict__ s, int64_t L, int32_t m, int64_t w0,
+                                          T* xa, T* xb) {
+    S va[kStatsR], vb[kStatsR];
+    stats_load(s, L, m, w0, va, vb);
+    const bool fin = stats_stage(va, vb, xa, xb);
+    return stats_tile_staged<T, S, RAW>(s, L, m, w0, xa, xb, fin);
+}
+
+
+// Whole-stream sweep: wave t of the grid takes tiles t, t + stride, ...
+// (stride = the grid's waves, sized to what the device holds at once) and
+// issues the loads of its next tile before computing the current one, so
+// the tile's FP64 work overlaps the next tile's HBM reads instead of
+// alternating with them.  emit(w0, nw) writes the tile's results out of
+// xa / xb.
+template <class T, class S, bool RAW, class Emit>
+__device__ __forceinline__ void stats_sweep(const S* __restrict__ s, int64_t L, int32_t m, T* xa,
+                                            T* xb, Emit emit) {
+    const int64_t ntiles = (L + kStatsTW - 1) / kStatsTW;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kStatsWaves;
+    // wave-uniform, so the tile bases and the loop stay in scalar registers
+    const int wid = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
+    int64_t t = static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid;
+    if (t >= ntiles) return;
+    S va[kStatsR], vb[kStatsR];
+    stats_load(s, L, m, t * kStatsTW, va, vb);
+    for (;;) {
+        const int64_t w0 = t * kStatsTW;
+        const bool fin = stats_stage(va, vb, xa, xb);
+        const int64_t tn = t + stride;
+        if (tn < ntiles) stats_load(s, L, m, tn * kStatsTW, va, vb);
+        const int nw = stats_tile_staged<T, S, RAW>(s, L, m, w0, xa, xb, fin);
+        emit(w0, nw);
+        if (tn >= ntiles) break;
+        t = tn;
+    }
 }
 
 __global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
@@ -269,17 +379,17 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_fp64_kernel(
     const int wid = threadIdx.x / kWave;
     float* xa = xa_all[wid];
     float* xb = xb_all[wid];
-    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
-    if (w0 >= L) return;
-    const int nw = stats_tile(s, L, m, w0, xa, xb);
+    stats_sweep<float, float, false>(s, L, m, xa, xb, [&](int64_t w0, int nw) {
+        float* pa = avg + w0 + lane;
+        float* pb = sd + w0 + lane;
 #pragma unroll 4
-    for (int v = 0; v < kStatsR; ++v) {
-        const int i = v * kWave + lane;
-        if (i < nw) {  // streamed out: not re-read by this kernel
-            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
-            __builtin_nontemporal_store(xb[stats_skew(i)], sd + w0 + i);
+        for (int v = 0; v < kStatsR; ++v) {
+            if (v * kWave + lane < nw) {  // streamed out: not re-read by this kernel
+                __builtin_nontemporal_store(xa[stats_slot(v, lane)], pa + v * kWave);
+                __builtin_nontemporal_store(xb[stats_slot(v, lane)], pb + v * kWave);
+            }
         }
-    }
+    });
 }
 
 // FP64 mean and variance of every window (the FFT ED's statistics).
@@ -292,17 +402,17 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_f64out_kernel(
     const int wid = threadIdx.x / kWave;
     double* xa = xa_all[wid];
     double* xb = xb_all[wid];
-    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
-    if (w0 >= L) return;
-    const int nw = stats_tile<double>(s, L, m, w0, xa, xb);
+    stats_sweep<double, float, false>(s, L, m, xa, xb, [&](int64_t w0, int nw) {
+        double* pa = avg + w0 + lane;
+        double* pb = var + w0 + lane;
 #pragma unroll 4
-    for (int v = 0; v < kStatsR; ++v) {
-        const int i = v * kWave + lane;
-        if (i < nw) {
-            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
-            __builtin_nontemporal_store(xb[stats_skew(i)], var + w0 + i);
+        for (int v = 0; v < kStatsR; ++v) {
+            if (v * kWave + lane < nw) {
+                __builtin_nontemporal_store(xa[stats_slot(v, lane)], pa + v * kWave);
+                __builtin_nontemporal_store(xb[stats_slot(v, lane)], pb + v * kWave);
+            }
         }
-    }
+    });
 }
 
 // MDTW_STATS_NONE: mean 0 and sigma 1 for every window (NaN for a window
@@ -316,17 +426,17 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_none_kernel(
     const int wid = threadIdx.x / kWave;
     float* xa = xa_all[wid];
     float* xb = xb_all[wid];
-    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
-    if (w0 >= L) return;
-    const int nw = stats_tile<float, float, true>(s, L, m, w0, xa, xb);
+    stats_sweep<float, float, true>(s, L, m, xa, xb, [&](int64_t w0, int nw) {
+        float* pa = avg + w0 + lane;
+        float* pb = sd + w0 + lane;
 #pragma unroll 4
-    for (int v = 0; v < kStatsR; ++v) {
-        const int i = v * kWave + lane;
-        if (i < nw) {
-            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
-            __builtin_nontemporal_store(xb[stats_skew(i)], sd + w0 + i);
+        for (int v = 0; v < kStatsR; ++v) {
+            if (v * kWave + lane < nw) {
+                __builtin_nontemporal_store(xa[stats_slot(v, lane)], pa + v * kWave);
+                __builtin_nontemporal_store(xb[stats_slot(v, lane)], pb + v * kWave);
+            }
         }
-    }
+    });
 }
 
 // FP64 stream (the reference's avg_std<double> for its ED driver): FP64 mean
@@ -342,20 +452,21 @@ __global__ __launch_bounds__(kStatsWaves * kWave) void stats_d_kernel(
     const int wid = threadIdx.x / kWave;
     double* xa = xa_all[wid];
     double* xb = xb_all[wid];
-    const int64_t w0 = (static_cast<int64_t>(blockIdx.x) * kStatsWaves + wid) * kStatsTW;
-    if (w0 >= L) return;
-    const int nw = stats_tile<double, double>(s, L, m, w0, xa, xb);
+    stats_sweep<double, double, false>(s, L, m, xa, xb, [&](int64_t w0, int nw) {
+        double* pa = avg + w0 + lane;
+        double* pb = sd + w0 + lane;
 #pragma unroll 4
-    for (int v = 0; v < kStatsR; ++v) {
-        const int i = v * kWave + lane;
-        if (i < nw) {
-            const double var = xb[stats_skew(i)];
-            // NaN (a non-finite sample) stays NaN
-            const double sg = !SIGMA ? var : (var > 0.0 ? sqrt(var) : (var == 0.0 ? 1.0 : var));
-            __builtin_nontemporal_store(xa[stats_skew(i)], avg + w0 + i);
-            __builtin_nontemporal_store(sg, sd + w0 + i);
+        for (int v = 0; v < kStatsR; ++v) {
+            if (v * kWave + lane < nw) {
+                const double var = xb[stats_slot(v, lane)];
+                // NaN (a non-finite sample) stays NaN
+                const double sg =
+                    !SIGMA ? var : (var > 0.0 ? sqrt(var) : (var == 0.0 ? 1.0 : var));
+                __builtin_nontemporal_store(xa[stats_slot(v, lane)], pa + v * kWave);
+                __builtin_nontemporal_store(sg, pb + v * kWave);
+            }
         }
-    }
+    });
 }
 
 // Statistics of listed windows: entry p is window pos[p] of length len[p]
@@ -485,6 +596,25 @@ __global__ void ref_finalize(const double* __restrict__ P1, const double* __rest
     }
 }
 
+// Blocks of a whole-stream sweep: as many as the device holds at once for
+// the kernel (at least one), fewer when the stream has fewer tiles.
+template <class K>
+unsigned sweep_blocks(K kernel, int64_t L) {
+    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
+    const int64_t nb = (L + per_block - 1) / per_block;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kStatsWaves * kWave, 0) !=
+            hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    return static_cast<unsigned>(std::max<int64_t>(
+        1, std::min<int64_t>(nb, static_cast<int64_t>(cus) * per_cu)));
+}
+
 }  // namespace
 
 int launch_stats_at(hipStream_t st, const float* s, int64_t n, const int32_t* len,
@@ -504,9 +634,7 @@ int launch_stats_fp64(hipStream_t st, const float* s, int64_t n, int32_t m, floa
                       float* sd) {
     const int64_t L = n - m + 1;
     if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
-    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
-    const int64_t nb = (L + per_block - 1) / per_block;
-    hipLaunchKernelGGL(stats_fp64_kernel, dim3(static_cast<unsigned>(nb)),
+    hipLaunchKernelGGL(stats_fp64_kernel, dim3(sweep_blocks(stats_fp64_kernel, L)),
                        dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -520,9 +648,7 @@ int launch_stats_f64out(hipStream_t st, const float* s, int64_t n, int32_t m, do
                         double* var) {
     const int64_t L = n - m + 1;
     if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
-    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
-    const int64_t nb = (L + per_block - 1) / per_block;
-    hipLaunchKernelGGL(stats_f64out_kernel, dim3(static_cast<unsigned>(nb)),
+    hipLaunchKernelGGL(stats_f64out_kernel, dim3(sweep_blocks(stats_f64out_kernel, L)),
                        dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, var);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -536,9 +662,7 @@ int launch_stats_none(hipStream_t st, const float* s, int64_t n, int32_t m, floa
                       float* sd) {
     const int64_t L = n - m + 1;
     if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
-    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
-    const int64_t nb = (L + per_block - 1) / per_block;
-    hipLaunchKernelGGL(stats_none_kernel, dim3(static_cast<unsigned>(nb)),
+    hipLaunchKernelGGL(stats_none_kernel, dim3(sweep_blocks(stats_none_kernel, L)),
                        dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
@@ -552,13 +676,11 @@ int launch_stats_d(hipStream_t st, const double* s, int64_t n, int32_t m, double
                    double* sd, bool sigma) {
     const int64_t L = n - m + 1;
     if (L <= 0) return MDTW_ERR_STREAM_TOO_SHORT;
-    constexpr int64_t per_block = static_cast<int64_t>(kStatsTW) * kStatsWaves;
-    const int64_t nb = (L + per_block - 1) / per_block;
     if (sigma)
-        hipLaunchKernelGGL(stats_d_kernel<true>, dim3(static_cast<unsigned>(nb)),
+        hipLaunchKernelGGL(stats_d_kernel<true>, dim3(sweep_blocks(stats_d_kernel<true>, L)),
                            dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
     else
-        hipLaunchKernelGGL(stats_d_kernel<false>, dim3(static_cast<unsigned>(nb)),
+        hipLaunchKernelGGL(stats_d_kernel<false>, dim3(sweep_blocks(stats_d_kernel<false>, L)),
                            dim3(kStatsWaves * kWave), 0, st, s, L, m, avg, sd);
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) {
