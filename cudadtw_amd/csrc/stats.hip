@@ -198,7 +198,12 @@ __device__ __forceinline__ bool stats_stage(const S (&va)[kStatsR], const S (&vb
 // x - c, F1 non-finite samples): pass 1 totals each lane's transitions, a
 // wave scan offsets them, pass 2 walks the lane's windows.  CHECK keeps the
 // non-finite bookkeeping (needed only when the tile holds such a sample).
-template <class T, class S, bool RAW, bool CHECK>
+// FULL (nw == kStatsTW, every tile but a stream's last): every lane has
+// kStatsR windows, so the loops run without a per-lane bound.  Either way a
+// lane's totals hold exactly its transitions r -> r + 1 with r + 1 < nw: the
+// exclusive scan is the inclusive scan minus the lane's own total, so that
+// total enters the rounding of the lane's own windows.
+template <class T, class S, bool RAW, bool CHECK, bool FULL>
 __device__ __forceinline__ void stats_windows(int32_t m, int nw, double c, double S1, double S2,
                                               double B1, double F1, T* xa, T* xb) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -214,9 +219,11 @@ __device__ __forceinline__ void stats_windows(int32_t m, int nw, double c, doubl
     double t1 = 0.0, t2 = 0.0, u1 = 0.0, u2 = 0.0, tf = 0.0;
     // r0 + v < nw as v < nv: one per-lane bound, compared with constants
     const int nv = nw - r0;
+#pragma unroll 4
     for (int v = 0; v < kStatsR; ++v) {
         const int r = r0 + v;
-        if (v + 1 < nv) {
+        // transition r -> r + 1 exists when r + 1 < nw
+        if (FULL ? (v + 1 < kStatsR || lane + 1 < kWave) : v + 1 < nv) {
             double fo = 0.0, fn = 0.0;
             const double xo = shifted(xa[stats_skew(r)], fo);
             const double xn = shifted(xb[stats_skew(r)], fn);
@@ -236,9 +243,10 @@ __device__ __forceinline__ void stats_windows(int32_t m, int nw, double c, doubl
     // Pass 2: the lane's windows; results overwrite their own LDS slots
     // after the slot's samples are read.
     const double inv_m = 1.0 / static_cast<double>(m);
+#pragma unroll 4
     for (int v = 0; v < kStatsR; ++v) {
         const int r = r0 + v;
-        if (v >= nv) break;
+        if (!FULL && v >= nv) break;
         const double sum1 = S1 + p1;
         const double sum2 = S2 + p2;
         const double mean = sum1 * inv_m;
@@ -300,32 +308,63 @@ __device__ __forceinline__ int stats_tile_staged(const S* __restrict__ s, int64_
 
     // First window of the tile: x[w0 .. w0 + m), from the staged tile where
     // it lies inside it (x[w0 + k] = xa[k] for k < kStatsTW).
+    // When the staged samples are all finite, the sums skip the non-finite
+    // test for them and count only the samples read from memory (m >
+    // kStatsTW); one non-finite sample there sends the window through the
+    // checked sums instead.  Finite samples give the same sums either way.
     double a1 = 0.0, a2 = 0.0, b1 = 0.0, f1 = 0.0;
-    auto first = [&](S xk) {
-        const bool fin = isfinite(xk);
-        const double d = fin ? static_cast<double>(xk) - c : 0.0;
+    auto add = [&](S xk) {
+        const double d = static_cast<double>(xk) - c;
         a1 += d;
         b1 += fabs(d);
         a2 = fma(d, d, a2);
-        f1 += fin ? 0.0 : 1.0;
     };
     const int mt = m < kStatsTW ? m : kStatsTW;
-    int k = lane;
-    for (; k < mt; k += kWave) first(static_cast<S>(xa[stats_skew(k)]));
-    for (; k < m; k += kWave) first(x[k]);
+    bool clean = false;
+    if (staged_finite) {
+        int fin = 1;
+        int k = lane;
+        for (; k < mt; k += kWave) add(static_cast<S>(xa[stats_skew(k)]));
+        for (; k < m; k += kWave) {
+            const S xk = x[k];
+            fin &= static_cast<int>(isfinite(xk));
+            add(xk);
+        }
+        clean = __all(fin);
+    }
+    if (!clean) {
+        a1 = a2 = b1 = 0.0;
+        auto add_checked = [&](S xk) {
+            const bool fin = isfinite(xk);
+            add(fin ? xk : static_cast<S>(c));  // contributes 0
+            f1 += fin ? 0.0 : 1.0;
+        };
+        int k = lane;
+        for (; k < mt; k += kWave) add_checked(static_cast<S>(xa[stats_skew(k)]));
+        for (; k < m; k += kWave) add_checked(x[k]);
+    }
     const double S1 = wave_sum_f64(a1);
     const double S2 = wave_sum_f64(a2);
     const double B1 = wave_sum_f64(b1);
-    const double F1 = wave_sum_f64(f1);  // non-finite samples of the first window
+    // non-finite samples of the first window
+    const double F1 = clean ? 0.0 : wave_sum_f64(f1);
 
     // Every sample a window of the tile reads is in the first window or
     // among the staged ones: when all of those are finite (the usual case),
     // the transitions skip the non-finite bookkeeping -- the same sums
     // otherwise, so the same results.
-    if (staged_finite && F1 == 0.0)
-        stats_windows<T, S, RAW, false>(m, nw, c, S1, S2, B1, F1, xa, xb);
-    else
-        stats_windows<T, S, RAW, true>(m, nw, c, S1, S2, B1, F1, xa, xb);
+    const bool fin = staged_finite && F1 == 0.0;
+    if (nw == kStatsTW) {
+        if (fin)
+            stats_windows<T, S, RAW, false, true>(m, nw, c, S1, S2, B1, F1, xa, xb);
+        else
+            stats_windows<T, S, RAW, true, true>(m, nw, c, S1, S2, B1, F1, xa, xb);
+    } else {
+        if (fin)
+            stats_windows<T, S, RAW, false, false>(m, nw, c, S1, S2, B1, F1, xa, xb);
+        else
+            stats_windows<T, S, RAW, true, false>(m, nw, c, S1, S2, B1, F1, xa, xb);
+    }
     stats_wave_fence();
     return nw;
 }
