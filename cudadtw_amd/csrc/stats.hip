@@ -325,6 +325,19 @@ __device__ __forceinline__ int stats_tile_staged(const S* __restrict__ s, int64_
         int fin = 1;
         int k = lane;
         for (; k < mt; k += kWave) add(static_cast<S>(xa[stats_skew(k)]));
+        // Samples past the tile come from memory: sixteen loads in flight at
+        // a time (one at a time, each waited for, cost a memory latency per
+        // sample), added in the same order.
+        for (; k + 15 * kWave < m; k += 16 * kWave) {
+            S xk[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) xk[u] = x[k + u * kWave];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                fin &= static_cast<int>(isfinite(xk[u]));
+                add(xk[u]);
+            }
+        }
         for (; k < m; k += kWave) {
             const S xk = x[k];
             fin &= static_cast<int>(isfinite(xk));
