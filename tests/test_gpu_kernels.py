@@ -58,6 +58,61 @@ def test_stats_fp64_vs_two_pass(engine, kind, m):
     assert rel.max() < 2e-7, rel.max()
 
 
+def _two_pass(s, wins, m):
+    """FP64 two-pass mean and population sigma of the listed windows."""
+    x = np.lib.stride_tricks.sliding_window_view(s.astype(np.float64), m)[wins]
+    mu = x.mean(axis=1)
+    return mu, np.sqrt(((x - mu[:, None]) ** 2).mean(axis=1))
+
+
+@pytest.mark.parametrize("m", [256, 1500])
+def test_stats_sweep_over_many_tiles(engine, m):
+    # More 1024-window tiles than the statistics kernel has resident waves
+    # (each wave then takes several tiles, the next one's loads in flight),
+    # a partial last tile, a first window longer than a tile (m = 1500),
+    # and two tiles holding a non-finite sample: a sample of windows from
+    # every part against FP64 two-pass statistics, and NaN exactly for the
+    # windows holding the planted samples.
+    n = (1 << 23) + 777
+    s = synth.random_walk(n, 21)
+    bad = [3_000_123, 6_500_000]
+    s[bad[0]] = np.nan
+    s[bad[1]] = np.inf
+    L = n - m + 1
+    avg, sd = engine.avg_std(dev(s), m, STATS_FP64)
+    a, d = host(avg), host(sd)
+    hit = np.zeros(L, bool)
+    for p in bad:
+        hit[max(0, p - m + 1):p + 1] = True
+    assert np.all(np.isnan(a[hit])) and np.all(np.isnan(d[hit]))
+    assert np.all(np.isfinite(a[~hit])) and np.all(np.isfinite(d[~hit]))
+    wins = np.unique(np.concatenate([np.arange(0, L, 997), np.arange(L - 3000, L),
+                                     np.arange(bad[0] - m - 40, bad[0] - m + 1),
+                                     np.arange(bad[1] + 1, bad[1] + 41)]))
+    wins = wins[~hit[wins]]
+    mu, sg = _two_pass(s, wins, m)
+    scale = np.abs(s[np.isfinite(s)]).max()
+    assert np.max(np.abs(a[wins] - mu)) < 1e-6 * scale
+    rel = np.abs(d[wins] - sg) / sg
+    assert rel.max() < 2e-7, rel.max()
+
+
+def test_stats_sweep_fp64_stream(engine):
+    # The FP64-stream kernel holds two workgroups per CU, so 2^22 windows
+    # are already several tiles per wave; tolerances as for the FP64 ED path.
+    m = 700
+    n = (1 << 22) + 9
+    s = synth.metal_like(n, 5, offset=7000.0).astype(np.float64)
+    s += 1e-3 * np.random.default_rng(5).standard_normal(n)
+    avg, sd = engine.avg_std_f64(dev(s, torch.float64), m)
+    a, d = host(avg), host(sd)
+    L = n - m + 1
+    wins = np.unique(np.concatenate([np.arange(0, L, 1009), np.arange(L - 2000, L)]))
+    mu, sg = _two_pass(s, wins, m)
+    np.testing.assert_allclose(a[wins], mu, rtol=1e-12)
+    np.testing.assert_allclose(d[wins], sg, rtol=1e-9)
+
+
 def test_stats_short_streams_and_constant_segments(engine):
     for n, m in ((11, 11), (12, 11), (300, 100)):
         s = synth.random_walk(n, 2)
