@@ -12,7 +12,9 @@
 // absolute level (the reference's global FP64 prefix loses ~1e-2 of sigma on
 // offset data; its FP32 finalize loses more, stats.cuh:40-52).
 // HBM traffic: read (2T + m)/T floats per window (the second read of each
-// sample mostly hits L2), write 8 bytes.
+// sample mostly hits L2), write 8 bytes.  The whole-stream kernels sweep the
+// tiles with as many waves as the device holds, each wave loading its next
+// tile while it computes the current one (stats_sweep).
 //
 // Non-finite samples (NaN, +-Inf), FP64 and raw modes: a window holding one
 // gets NaN for its mean and sigma, which makes every distance and bound of it
@@ -163,10 +165,9 @@ __device__ __forceinline__ void stats_load(const S* __restrict__ s, int64_t L, i
         }
         return;
     }
-    // The stream's last tile: sample indices clamped to last, written as
-    // v * 64 + min(lane, bound_v) with a wave-uniform bound_v (here
-    // last - w0 < m + kStatsTW fits an int; an index min(lane, negative) + ...
-    // still lands on last itself).
+    // The stream's last tile: index min(v * 64 + lane, last - w0), written
+    // as v * 64 + min(lane, b) with the wave-uniform b = last - w0 - v * 64
+    // (last - w0 < m + kStatsTW fits an int; a negative b gives last - w0).
     const int la = static_cast<int>(last - w0);
 #pragma unroll
     for (int v = 0; v < kStatsR; ++v) {
