@@ -203,7 +203,7 @@ __global__ __launch_bounds__(64) void ldtw_kernel(
 
 int launch_ldtw(hipStream_t st, int cus, const BandArgs& a, int32_t w, int32_t metric) {
     if (!a.idx_count_from_ctrl && a.count <= 0) return MDTW_OK;
-    BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent};
+    BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent, 0, 0};
     // Persistent: 32 one-wave workgroups per CU, 8 waves per SIMD (the
     // kernel's register occupancy), which hides the per-step stream gathers
     // of one wave behind the others' walks (C3 |Q| = 512 / 1024 searches

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void thr_scan_kernel(int32_t* __restrict__ bc,
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-        int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        int t = static_cast<int>(threadIdx.x) >= off ? sh[threadIdx.x - off] : 0;
         __syncthreads();
         sh[threadIdx.x] += t;
         __syncthreads();
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void thr_scatter_kernel(const float* __restric
         if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = __popcll(mask);
         __syncthreads();
         int off = base;
-        for (int wv = 0; wv < threadIdx.x / 64; ++wv) off += sh[wv];
+        for (int wv = 0; wv < static_cast<int>(threadIdx.x / 64); ++wv) off += sh[wv];
         if (keep) outi[off + r] = in[i];
         base += sh[0] + sh[1] + sh[2] + sh[3];
         __syncthreads();
