@@ -291,9 +291,15 @@ BandLaunchFn band_lookup(int w, bool l2) {
         return l2 ? band_table_7(w) : band_table_11(w);
     }
     if (w < 30) return band_table_0(w, l2);
-    if (w < 60) return band_table_1(w, l2);
-    if (w < 90) return band_table_2(w, l2);
-    return band_table_3(w, l2);
+    if (w < 40) return band_table_1(w, l2);
+    if (w < 50) return band_table_12(w, l2);
+    if (w < 60) return band_table_13(w, l2);
+    if (w < 70) return band_table_2(w, l2);
+    if (w < 80) return band_table_14(w, l2);
+    if (w < 90) return band_table_15(w, l2);
+    if (w < 98) return band_table_3(w, l2);
+    if (w < 106) return band_table_16(w, l2);
+    return band_table_17(w, l2);
 }
 
 }  // namespace

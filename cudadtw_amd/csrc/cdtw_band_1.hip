@@ -1,5 +1,5 @@
 // cdtw_band_1.hip -- exact-W instantiations of the register-band cDTW kernel
-// (cdtw_band.hpp) for W in [30, 59]; split across units so they compile in
+// (cdtw_band.hpp) for W in [30, 39]; split across units so they compile in
 // parallel.
 #include "cdtw_band.hpp"
 
@@ -16,6 +16,6 @@ BandLaunchFn band_pick(int w, bool l2) {
 }
 }  // namespace
 
-BandLaunchFn band_table_1(int w, bool l2) { return band_pick<30, 59>(w, l2); }
+BandLaunchFn band_table_1(int w, bool l2) { return band_pick<30, 39>(w, l2); }
 
 }  // namespace mdtw

@@ -1,5 +1,5 @@
 // cdtw_band_2.hip -- exact-W instantiations of the register-band cDTW kernel
-// (cdtw_band.hpp) for W in [60, 89]; split across units so they compile in
+// (cdtw_band.hpp) for W in [60, 69]; split across units so they compile in
 // parallel.
 #include "cdtw_band.hpp"
 
@@ -16,6 +16,6 @@ BandLaunchFn band_pick(int w, bool l2) {
 }
 }  // namespace
 
-BandLaunchFn band_table_2(int w, bool l2) { return band_pick<60, 89>(w, l2); }
+BandLaunchFn band_table_2(int w, bool l2) { return band_pick<60, 69>(w, l2); }
 
 }  // namespace mdtw
