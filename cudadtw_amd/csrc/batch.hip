@@ -8,10 +8,11 @@
 // trip) on P concurrent pipelines (streams), each query followed by an
 // on-device greedy top-k selection, and finally copies all results to pinned
 // host memory.  The whole sequence is captured once into a hipGraph and
-// replayed: one graph launch per batch search.  Queries are assigned to
-// pipelines longest-first (largest L * cells(M, W) to the least-loaded
-// pipeline), so the pipelines end together; their kernels overlap on the
-// device, which fills the tails of one query's kernels with another's work.
+// replayed: one graph launch per batch search.  Queries heavy enough to fill
+// the device on their own run one after another on the first pipeline; the
+// others are assigned longest-first (largest L * cells(M, W) to the
+// least-loaded pipeline), so their kernels overlap on the device and fill
+// the tails of one query's kernels with another's work.
 //
 // Launch shapes are picked per (|Q|, band) bucket by mdtw_batch_tune: the
 // persistent DP grid (windows per workgroup pass times workgroups), the
@@ -37,6 +38,9 @@ namespace {
 
 constexpr int kDefaultPipelines = 4;
 constexpr int kMaxPipelines = 16;
+// Band cells (times the per-cell cost class) of a query from which it runs
+// serialized with the other such queries (capture): 2^35, about 2 ms of DP.
+constexpr double kHeavyCells = 34359738368.0;
 
 int64_t band_cells(int32_t m, int32_t w) {
     return static_cast<int64_t>(m) * (2 * w + 1) - static_cast<int64_t>(w) * (w + 1);
@@ -249,7 +253,17 @@ int capture(mdtw_batch* b, const float* s, int64_t n, int32_t metric, int32_t k,
         rc = hip_ok(hipStreamWaitEvent(b->streams[p], b->fork, 0));
     for (int32_t i : order) {
         if (rc != MDTW_OK) break;
-        const int p = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+        // A query whose work alone keeps the device busy for milliseconds
+        // runs on pipeline 0, one after another: side by side such queries'
+        // grid-filling persistent kernels only compete (C4's sample of 32
+        // queries on a 2^26 stream: 13.09 s on four pipelines, 12.57 s one
+        // after another).  Lighter ones go to the least-loaded pipeline,
+        // where their short kernels fill each other's gaps (and the heavy
+        // queries' tails).
+        const int p = cost[i] >= kHeavyCells
+                          ? 0
+                          : static_cast<int>(std::min_element(load.begin(), load.end()) -
+                                             load.begin());
         load[p] += cost[i];
         SearchPlan pl = b->plan(i, b->streams[p], s, n, metric, k, excl[i], b->d_ctrls + i,
                                 b->slot_ws[p], b->shape_of(i));

@@ -587,6 +587,9 @@ int band_launch(hipStream_t st, unsigned grid, const BandArgs& a) {
         if constexpr (band_pair_tight(W))
             hipLaunchKernelGGL((band_pair_kernel2<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
                                a.idx, a.out, a.res, a.ctrl, sc);
+        else if constexpr (band_pair_three(W))
+            hipLaunchKernelGGL((band_pair_kernel3<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
+                               a.idx, a.out, a.res, a.ctrl, sc);
         else
             hipLaunchKernelGGL((band_pair_kernel<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
                                a.idx, a.out, a.res, a.ctrl, sc);

@@ -55,6 +55,14 @@ constexpr int kPairWaves = 2;  // one pair per workgroup
 // cell loop: measured at M = 1024, 15.6..15.8 T cells/s at W = 228..240
 // (13.1 at 236) against 8 at one wave.
 constexpr bool band_pair_tight(int w) { return w > 225; }
+// From W = 138 a half band and the rest take a few more than 168 registers,
+// and left to itself the compiler drops to two waves per SIMD (W = 137 at
+// 17.4 T cells/s, 138 at 14.5).  Up to W = 158 those widths are compiled for
+// three waves (band_pair_kernel3), spilling a few values outside the cell
+// loop: measured at M = 1024, 2^22 windows, W = 138 / 145 / 150 / 155 / 158
+// at 17.6 / 17.7 / 17.8 / 16.6 / 16.5 T cells/s against 14.2 / 14.3 / 15.1 /
+// 14.9 / 15.4 at two (W = 159 even, beyond slower).
+constexpr bool band_pair_three(int w) { return w >= 138 && w <= 158; }
 
 struct PairLds {
     float lo_last[kWave];   // lower: E[W] of its column (the upper's next first-cell input)
@@ -279,6 +287,12 @@ __global__ __launch_bounds__(kPairWaves * kWave) void band_pair_kernel(MDTW_PAIR
 template <int W, bool L2>
 __global__ __launch_bounds__(kPairWaves * kWave) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 band_pair_kernel2(MDTW_PAIR_KERNEL_ARGS) {
+    pair_kernel_body<W, L2>(s, avg, sd, qx, idx, out, res, ctrl, a);
+}
+// Compiled for three waves per SIMD (168 registers per lane).
+template <int W, bool L2>
+__global__ __launch_bounds__(kPairWaves * kWave) __attribute__((amdgpu_waves_per_eu(3, 3))) void
+band_pair_kernel3(MDTW_PAIR_KERNEL_ARGS) {
     pair_kernel_body<W, L2>(s, avg, sd, qx, idx, out, res, ctrl, a);
 }
 #undef MDTW_PAIR_KERNEL_ARGS
