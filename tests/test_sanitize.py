@@ -13,6 +13,7 @@
 Any sanitizer report fails the run (halt_on_error, no recovery).
 """
 import os
+import re
 import shutil
 import subprocess
 
@@ -145,8 +146,14 @@ def test_oracle_under_asan_ubsan(tmp_path):
     assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr
 
 
-OBJS = ["engine", "search", "stream", "batch", "noalloc", "ed_fft", "select", "stats", "bounds",
-        "cdtw", "ldtw", "ed", "backward"] + [f"cdtw_band_{i}" for i in range(12)]
+def _library_units():
+    """The translation units of libmdtw.so, as the library's Makefile lists them."""
+    text = open(os.path.join(ROOT, "cudadtw_amd", "csrc", "Makefile")).read()
+    m = re.search(r"^SRCS\s*:=(.*?)(?:\n(?!\s)|\Z)", text, re.S | re.M)
+    return [f[:-4] for f in m.group(1).replace("\\", " ").split() if f.endswith(".hip")]
+
+
+OBJS = _library_units()
 SANITIZED = ["engine", "search", "stream", "batch", "noalloc", "ed_fft"]
 
 
