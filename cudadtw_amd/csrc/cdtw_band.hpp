@@ -47,7 +47,7 @@ constexpr int kBandSlideMax = 54;
 // per SIMD); W = 113..114 keep the 2W + 1 band values in vector registers,
 // one column at a time (two waves per SIMD), W = 115..kBandTailMaxW in vector
 // registers plus an LDS tail (below), and the bands from kBandPairMinW to
-// kBandWideMaxW are split over two waves (cdtw_pair.hpp).
+// kBandWideMaxW are split over two or three waves (cdtw_pair.hpp).
 constexpr int kBandMaxW = 112;
 constexpr int kBandWideMaxW = 240;
 // Bands W = 115..kBandTailMaxW keep band values 0..kBandTailR-1 in vector
@@ -575,14 +575,22 @@ bool band_lb_pays() {
 }
 
 // `grid` counts workgroups of kBandWaves one-lane waves (256 windows per
-// pass); a split band takes four pairs of waves for each (the same windows per
-// pass, twice the waves: two or three of them fit a SIMD, against one or two
-// one-lane waves of these widths).
+// pass); a split band takes four pairs or trios of waves for each (the same
+// windows per pass, twice or three times the waves: two or three of them fit
+// a SIMD, against one or two one-lane waves of these widths).
 template <int W, bool L2>
 int band_launch(hipStream_t st, unsigned grid, const BandArgs& a) {
     BandScalars sc{a.m, a.idx_count_from_ctrl, a.count, a.base, a.stride, a.mode, a.persistent,
                    0, a.qx_len};
-    if constexpr (W >= kBandPairMinW) {
+    if constexpr (band_trio(W)) {
+        const dim3 g(grid * kBandWaves), b(kTrioWaves * kWave);
+        if constexpr (band_trio_three(W))
+            hipLaunchKernelGGL((band_trio_kernel3<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
+                               a.idx, a.out, a.res, a.ctrl, sc);
+        else
+            hipLaunchKernelGGL((band_trio_kernel<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
+                               a.idx, a.out, a.res, a.ctrl, sc);
+    } else if constexpr (W >= kBandPairMinW) {
         const dim3 g(grid * kBandWaves), b(kPairWaves * kWave);
         if constexpr (band_pair_tight(W))
             hipLaunchKernelGGL((band_pair_kernel2<W, L2>), g, b, 0, st, a.s, a.avg, a.sd, a.qx,
